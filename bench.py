@@ -1,0 +1,186 @@
+"""Benchmark: 30-qubit complex64 plain-MUX2 pass bandwidth on one MI355X.
+
+Workload: the 30-qubit layer circuit (U3 on every qubit with angles U[0, 2pi), then CX on a seeded
+matching in which each of qubits 26..29 is paired with one of 0..25; depth 20, seed 1), compiled
+by this project's compiler for the MI355X tile (2^13 amplitudes, up to 7 gathered high qubits).
+A step applies every pass of the compiled circuit once to the 8 GiB state resident in HBM (the
+state is a valid input for the next step, so nothing is re-initialised inside the timed region).
+
+value = passes x 2 x 2^30 x 8 B / time per step (the reference's sweep-bandwidth definition,
+analysis/make_tables.py:320-321).  roofline: the pass kernel's algorithmic bytes per launch over
+its mean duration from HIP events on the launch stream.  cpu_baseline: the numpy golden model
+(oracle/model.py, the reference's CPU path) on the 20-qubit workload, one pinned core, in a
+child process.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0
+
+
+def cpu_baseline(max_seconds=30.0):
+    """Child-process entry: time the golden model on the 20-qubit workload (reference stream:
+    P = 8 slices of 2^17, k = 12, MAXH 4 -- the reference runtime's plan for 20 qubits)."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    os.environ["MKL_NUM_THREADS"] = "1"
+    core = sorted(os.sched_getaffinity(0))[0]
+    os.sched_setaffinity(0, {core})
+    import numpy as np
+    import __graft_entry__ as ge
+    ge.load_package()
+    from u55c_statevector_30q_amd import circuit, compiler
+    from oracle import model
+    qc = circuit.workload_cpu20()
+    t0 = time.perf_counter()
+    prog, fmap, st = compiler.compile_circuit(qc, 17, 12, max_high=4)
+    t_compile = time.perf_counter() - t0
+    times = []
+    t_start = time.perf_counter()
+    for rep in range(4):
+        a = time.perf_counter()
+        model.run_program(prog, 20, 8, fmap, ctype=np.complex64)
+        times.append(time.perf_counter() - a)
+        if time.perf_counter() - t_start > max_seconds and rep >= 1:
+            break
+    timed = times[1:] if len(times) > 1 else times
+    med = float(np.median(timed))
+    sweeps = st.n_passes + st.n_remaps
+    gbs = sweeps * 2 * (1 << 20) * 8 / med / 1e9
+    return dict(value=round(gbs, 4), unit="GB/s", cores=1, kind="port",
+                sample=(f"golden model (numpy c64) on the 20-qubit workload (depth 20, seed 0): "
+                        f"{st.n_ops} ops, {st.n_passes} passes + {st.n_remaps} remaps at P=8, L=17, k=12; "
+                        f"median of {len(timed)} runs after 1 warm-up = {med:.3f} s; compile {t_compile:.3f} s; "
+                        f"pinned to core {core} of {os.cpu_count()}"))
+
+
+def _run_cpu_child():
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return dict(error=r.stderr[-400:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-judge", action="store_true")
+    ap.add_argument("--cpu-baseline-child", action="store_true")
+    a = ap.parse_args()
+    if a.cpu_baseline_child:
+        print(json.dumps(cpu_baseline()))
+        return
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import __graft_entry__ as ge
+    ge.load_package()
+    from u55c_statevector_30q_amd import circuit, compiler, libsvx
+
+    caps = libsvx.init(0)
+    n = 30
+    qc = circuit.workload_gpu30()
+    t0 = time.perf_counter()
+    prog, fmap, st = compiler.compile_gpu(qc)
+    t_compile = time.perf_counter() - t0
+    assert all(kind == "segment" for kind, _ in prog), "the 30-qubit workload compiles without remaps"
+    handles = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
+    n_launch = sum(libsvx.prepared_info(h)[1] for h in handles)
+    state = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
+    libsvx.zero_state(state)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        for h in handles:
+            libsvx.run_prepared(state, h)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    ms_per_step = t_total / a.steps * 1e3
+    bytes_per_pass = 2 * (1 << n) * 8
+    value = n_launch * bytes_per_pass / (t_total / a.steps) / 1e9
+
+    # per-pass durations: each pass as its own prepared segment, HIP events on the launch stream
+    single = [libsvx.prepare(seg, libsvx.C64, n) for _, w in prog for seg in compiler.split_passes(w)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in single]
+    for (e0, e1), h in zip(evs, single):
+        e0.record(stream)
+        libsvx.run_prepared(state, h)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    per_pass_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    mean_ms = float(np.mean(per_pass_ms))
+    achieved = bytes_per_pass / (mean_ms * 1e-3) / 1e9
+    per_pass_gbs = [bytes_per_pass / (t * 1e-3) / 1e9 for t in per_pass_ms]
+
+    judge = None
+    if not a.no_judge:
+        s64 = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
+        s128 = torch.empty(1 << n, dtype=torch.complex128, device="cuda:0")
+        for s in (s64, s128):
+            libsvx.zero_state(s)
+            for _, w in prog:
+                libsvx.run_segment(s, w)
+        torch.cuda.synchronize()
+        judge = libsvx.fidelity_f64(s128, s64)
+        norm, _ = libsvx.reduce_f64(s64)
+        judge["norm_c64"] = norm
+        del s64, s128
+        torch.cuda.empty_cache()
+
+    cpu = None if a.no_cpu else _run_cpu_child()
+    out = {
+        "metric": "30q c64 pass GB/s (% of 8 TB/s)",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "u3cx_30q_d20_s1: 30 qubits, complex64, depth 20, seed 1, U3 + CX matching "
+                               "(26..29 paired with 0..25)",
+                   "n_qubits": n, "ops": st.n_ops, "passes": st.n_passes, "launches": n_launch,
+                   "tile_k": compiler.GPU_K, "compile_s": round(t_compile, 3)},
+        "frac_of_peak": round(value / PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pass<float,13>", "bytes_per_launch": bytes_per_pass,
+                     "mean_launch_ms": round(mean_ms, 4)},
+        "per_pass_gbs": {"min": round(min(per_pass_gbs), 1), "median": round(float(np.median(per_pass_gbs)), 1),
+                         "max": round(max(per_pass_gbs), 1),
+                         "n_at_or_above_70pct": int(sum(g >= 0.7 * PEAK_GBS for g in per_pass_gbs)),
+                         "n": len(per_pass_gbs)},
+        "judge_vs_c128": judge,
+        "cpu_baseline": cpu,
+        "device": caps.arch.decode(),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+/* libsvx -- MI355X (gfx950) statevector engine for the QSV2 MUX2 record stream.
+ *
+ * C ABI: plain pointers and sizes, no torch or HIP types in the signatures.  `state` and the other
+ * buffer arguments are device pointers (e.g. a torch tensor's data_ptr()); `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  Every function returns 0 on success or a
+ * negative SVX_E_* code; svx_last_error() returns the calling thread's last message.  Launches are
+ * asynchronous on `stream` unless the comment says the call synchronises.
+ *
+ * dtype: SVX_C64 = complex64 (interleaved float32 re, im -- the numpy / torch layout of the
+ * reference, kernels/sv/sv_engine.cpp:4-5), SVX_C128 = complex128 (the error judge).
+ *
+ * Ownership (as in the reference, host/sv/runtime.py:173-203): the caller owns every state /
+ * scratch buffer; the library never frees caller memory.  The library copies each QSV2 stream it
+ * is given into its own device-resident descriptor cache before returning, so the host array may
+ * be reused at once; a stream already seen is not uploaded again (replays launch kernels only).
+ */
+#ifndef SVX_H
+#define SVX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVX_C64 0
+#define SVX_C128 1
+
+#define SVX_OK 0
+#define SVX_E_ARG (-1)      /* bad argument (maps to ValueError) */
+#define SVX_E_MAGIC (-2)    /* stream does not start with the QSV2 magic (AssertionError, host/sv/model.py:9) */
+#define SVX_E_FORMAT (-3)   /* malformed stream: truncated, bad opcode, window out of range (ValueError) */
+#define SVX_E_CAPACITY (-4) /* tile / slice does not fit this device (RuntimeError, host/sv/runtime.py:220-221) */
+#define SVX_E_HIP (-5)      /* HIP runtime error (RuntimeError) */
+#define SVX_E_NODEV (-6)    /* no usable gfx950 device (RuntimeError) */
+
+typedef struct svx_caps {
+    int32_t device;          /* HIP device ordinal */
+    int32_t cu_count;        /* compute units */
+    int64_t lds_per_block;   /* bytes of LDS one workgroup may use */
+    int64_t hbm_bytes;       /* total device memory */
+    int32_t wavefront;       /* lanes per wavefront */
+    int32_t max_k_c64;       /* largest tile (log2 amplitudes) the pass kernel holds, complex64 */
+    int32_t max_k_c128;      /* same, complex128 */
+    int32_t max_high;        /* largest number of gathered high qubits per pass */
+    char arch[64];           /* gcnArchName */
+} svx_caps;
+
+/* Library / device set-up.  No reference counterpart (XRT device + xclbin load,
+ * host/sv/runtime.py:93-97); fills `out` when non-NULL. */
+int svx_init(int32_t device, svx_caps *out);
+
+/* Run one QSV2 segment on `n_slices` contiguous slices of `n_slice_amps` = 2^L amplitudes each
+ * (every slice runs the same segment independently).  Replaces the reference's run call
+ * `sv_engine(st, ins_off)` started on every slice's compute unit and waited on
+ * (kernels/sv/sv_engine.cpp:302, host/sv/runtime.py:227-233; semantics host/sv/model.py:59-71).
+ * Accepts reference streams (k <= 12, m <= 4) and retargeted ones (k <= max_k, m <= max_high;
+ * a pass with m > 4 carries an 8-word [0x48, h4..h10] record after its header). */
+int svx_run_segment(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t dtype,
+                    const int32_t *qsv2, int64_t n_words, void *stream);
+
+/* The reference kernel ABI verbatim: `st` is one slice buffer holding 2^L amplitudes followed by
+ * the instruction region starting at 256-bit word `ins_off` (kernels/sv/sv_engine.cpp:290-302,
+ * host/sv/runtime.py:218-233).  Reads the stream from device memory (synchronises `stream` once to
+ * fetch it), then runs it like svx_run_segment with one slice, complex64. */
+int svx_engine(void *st, int64_t ins_off, void *stream);
+
+/* Validate and upload a segment once; returns a handle for svx_run_prepared (no reference
+ * counterpart: the reference re-uploads instructions per launch, host/sv/runtime.py:218-225). */
+int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle);
+/* Launch a prepared segment: kernels only, no allocation or copy (safe under stream capture). */
+int svx_run_prepared(void *state, int64_t handle, void *stream);
+/* Number of passes (kernel launches) of a prepared segment, and the kernel index of pass p. */
+int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches);
+
+/* Qubit permutation perm[new_bit] = old_bit of a 2^n state, out of place (dst != src), bit-exact.
+ * Replaces the reference's remap paths: model.remap (host/sv/model.py:74-78), the host transpose
+ * of local qubits (host/sv/runtime.py:261-268) and the sv_swap bank exchange
+ * (host/sv/runtime.py:235-259, kernels/sv/sv_swap.cpp:10) -- any permutation in one HBM sweep. */
+int svx_permute(void *dst, const void *src, int32_t n, const int32_t *perm_new_to_old,
+                int32_t dtype, void *stream);
+
+/* The reference swap kernel's contract (kernels/sv/sv_swap.cpp:10): exchange nwords 64-byte words
+ * a[offa..offa+nwords) <-> b[offb..).  Ranges must not overlap. */
+int svx_swap_words(void *a, void *b, int64_t offa, int64_t offb, int64_t nwords, void *stream);
+
+/* |0...0>: every amplitude 0, amplitude 0 = 1 (host/sv/runtime.py:205-217, device-side zeroing
+ * pass + 32-byte write).  A fill kernel, no hipMemset (safe under stream capture). */
+int svx_zero_state(void *state, int32_t n, int32_t dtype, void *stream);
+
+/* Product state prod_q (a_q |0> + b_q |1>); amps = n x 4 doubles (a.re, a.im, b.re, b.im) per
+ * qubit q.  Computed in FP64 on the device and rounded once.  No reference counterpart (the
+ * reference always starts from |0...0>). */
+int svx_product_state(void *state, int32_t n, int32_t dtype, const double *amps, void *stream);
+
+/* FP64 reductions (the reference samples from FP32 chunk sums accumulated in FP64,
+ * host/sv/runtime.py:47-80): norm = sum |a|^2 and, when n_low > 0, the 2^n_low marginal
+ * probabilities of logical qubits 0..n_low-1 located at physical bits fmap[0..n_low-1]
+ * (out_marginals[x], bit i of x = logical qubit i).  Deterministic order; synchronises `stream`;
+ * outputs are host pointers (out_marginals may be NULL when n_low = 0). */
+int svx_reduce_f64(const void *state, int32_t n, int32_t dtype, const int32_t *fmap, int32_t n_low,
+                   double *out_norm, double *out_marginals, void *stream);
+
+/* Judge metrics in FP64 (tests/functional_suite.py:90-94, bench/fpga/sv_maxwidth_ref.py:15-25):
+ * out[0] = F = |<a|b>|^2 / (<a|a><b|b>), out[1] = 1-F = d^2 - d^4/4 after global-phase alignment
+ * of the normalised states (no cancellation), out[2] = max |b/|b| - ph a/|a||, out[3] = d.
+ * `a` is the reference.  Synchronises `stream`. */
+int svx_fidelity_f64(const void *a, int32_t dta, const void *b, int32_t dtb, int32_t n, double *out,
+                     void *stream);
+
+/* Copy / convert between dtypes (c128 -> c64 rounds once; c64 -> c128 exact). */
+int svx_convert(void *dst, int32_t dt_dst, const void *src, int32_t dt_src, int64_t n_amps, void *stream);
+
+int svx_sync(void *stream);
+const char *svx_last_error(void);
+const char *svx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

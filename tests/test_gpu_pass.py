@@ -1,0 +1,220 @@
+"""Parity of the HIP pass kernel (libsvx svx_run_segment) with the CPU golden model.
+
+Tolerances: the complex128 kernel must match the complex128 oracle to 1e-12 (FMA contraction is
+the only difference); the complex64 kernel is judged like the reference judges c64 states
+(bench/fpga/sv_maxwidth_ref.py:15-25): fidelity >= 1 - 1e-6 and max|da| <= 2e-5 against the
+complex128 oracle run of the same records.  Ops that only move data (exact X, controlled exact X)
+must be bit-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import judge, model
+from oracle.isa import MAGIC, OP_MUX2
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _lib():
+    from u55c_statevector_30q_amd import libsvx
+    return libsvx
+
+
+def _rand_state(rng, n, dtype=np.complex128):
+    s = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return (s / np.linalg.norm(s)).astype(dtype)
+
+
+def _rand_u(rng):
+    z = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def _mat(rng, kind):
+    if kind == "u":
+        return _rand_u(rng)
+    if kind == "i":
+        return np.eye(2)
+    if kind == "x":
+        return np.array([[0, 1], [1, 0]])
+    if kind == "d":
+        return np.diag(np.exp(1j * rng.uniform(0, 6.28, 2)))
+    if kind == "p":
+        return np.diag([1, np.exp(1j * rng.uniform(0, 6.28))])
+    raise ValueError(kind)
+
+
+OP_KINDS = [("u", "u"), ("d", "d"), ("x", "x"), ("i", "u"), ("i", "d"), ("i", "p"), ("i", "x"), ("u", "d"),
+            ("p", "x"), ("i", "i")]
+
+
+def _op_words(t, cmask, M0, M1):
+    fl = np.zeros(16, dtype=np.float32)
+    for i, M in enumerate((M0, M1)):
+        f = np.asarray(M, dtype=np.complex64).reshape(-1)
+        fl[8 * i:8 * i + 8:2] = f.real
+        fl[8 * i + 1:8 * i + 8:2] = f.imag
+    return [OP_MUX2, t, cmask, 0, 0, 0, 0, 0] + [int(x) for x in fl.view(np.int32)]
+
+
+def random_segment(rng, L, k, n_passes, max_ops=12, kinds=OP_KINDS, max_m=4, ctrl_p=0.3):
+    w = [MAGIC, L, n_passes, k, 0, 0, 0, 0]
+    for _ in range(n_passes):
+        m = int(rng.integers(0, min(max_m, k) + 1))
+        hi = sorted(rng.choice(list(range(k - m, L)), size=m, replace=False).tolist()) if m else []
+        h = hi + [0] * (11 - len(hi))
+        nops = int(rng.integers(1, max_ops + 1))
+        w += [m, h[0], h[1], nops, h[2], h[3], 0, 0]
+        if m > 4:
+            w += [0x48] + h[4:11]
+        for _ in range(nops):
+            t = int(rng.integers(0, k))
+            cm = 0
+            for b in range(k):
+                if b != t and rng.random() < ctrl_p:
+                    cm |= 1 << b
+            a, b = kinds[int(rng.integers(len(kinds)))]
+            w += _op_words(t, cm, _mat(rng, a), _mat(rng, b))
+    return (np.asarray(w, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+
+
+def run_hip(state_np, words, P=1):
+    torch = _torch()
+    t = torch.from_numpy(state_np.copy()).to("cuda")
+    _lib().run_segment(t, words, n_slices=P)
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _check_c64(ref128, got64, what):
+    m = judge.metrics(ref128, got64)
+    assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 2e-5, (what, m)
+
+
+CASES = [(4, 3, 1), (6, 4, 1), (8, 5, 2), (9, 6, 1), (10, 6, 1), (10, 10, 1), (11, 8, 2), (12, 12, 1),
+         (13, 9, 4), (14, 12, 1), (14, 13, 1), (15, 11, 2), (16, 12, 1), (18, 13, 1), (20, 12, 1), (20, 14, 1)]
+
+
+@pytest.mark.parametrize("n,k,P", CASES)
+def test_random_segments_c128_exact_and_c64(gpu, n, k, P):
+    rng = np.random.default_rng(1000 * n + 10 * k + P)
+    L = n - (P.bit_length() - 1)
+    k = min(k, L)
+    max_m = min(8, k) if k >= 8 else min(4, k)
+    seg = random_segment(rng, L, k, n_passes=3, max_m=max_m)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), P, seg, np.complex128)
+    if k <= 13:
+        got128 = run_hip(s0, seg, P)
+        assert np.max(np.abs(got128 - ref)) < 1e-12, (n, k, P)
+    got64 = run_hip(s0.astype(np.complex64), seg, P)
+    _check_c64(ref, got64, (n, k, P))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8])
+def test_every_target_and_control_small_tiles(gpu, k):
+    """Every target position and control pattern of small tiles (k <= 8), the case the reference
+    found only on hardware (docs/BASELINE_AUDIT.md:61-67)."""
+    rng = np.random.default_rng(k)
+    for n in (k + 2, 12):
+        if n < k:
+            continue
+        w = [MAGIC, n, 0, k, 0, 0, 0, 0]
+        npass = 0
+        for t in range(k):
+            ops = []
+            for cm in ([0] + [1 << b for b in range(k) if b != t] + [((1 << k) - 1) & ~(1 << t)]):
+                for a, b in (("u", "u"), ("i", "u"), ("i", "x"), ("i", "p"), ("u", "d")):
+                    ops.append(_op_words(t, cm, _mat(rng, a), _mat(rng, b)))
+            for i in range(0, len(ops), 64):
+                chunk = ops[i:i + 64]
+                m = min(2, k)
+                hi = sorted(rng.choice(list(range(k - m, n)), size=m, replace=False).tolist())
+                h = hi + [0] * (4 - len(hi))
+                w += [m, h[0], h[1], len(chunk), h[2], h[3], 0, 0]
+                for o in chunk:
+                    w += o
+                npass += 1
+        w[2] = npass
+        seg = (np.asarray(w, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+        s0 = _rand_state(rng, n)
+        ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+        got = run_hip(s0, seg)
+        assert np.max(np.abs(got - ref)) < 1e-11, (k, n)
+
+
+@pytest.mark.parametrize("n,k", [(8, 4), (12, 10), (16, 13), (22, 13)])
+def test_permutation_ops_bit_exact(gpu, n, k):
+    """X and controlled-X move data only: the c64 kernel must equal the c64 golden model bit for bit."""
+    rng = np.random.default_rng(n)
+    seg = random_segment(rng, n, k, n_passes=4, kinds=[("x", "x"), ("i", "x")], max_m=min(8, k) if k >= 8 else 4)
+    s0 = _rand_state(rng, n, np.complex64)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex64)
+    got = run_hip(s0, seg)
+    assert np.array_equal(got, ref)
+
+
+def test_zero_segment_and_errors(gpu):
+    from u55c_statevector_30q_amd import libsvx
+    torch = _torch()
+    rng = np.random.default_rng(0)
+    s0 = _rand_state(rng, 12, np.complex64)
+    got = run_hip(s0, model.zero_segment(12, 12))
+    assert not got.any()
+    t = torch.zeros(1 << 12, dtype=torch.complex64, device="cuda")
+    bad = model.zero_segment(12, 12).copy()
+    bad[0] = 7
+    with pytest.raises(AssertionError):
+        libsvx.run_segment(t, bad)
+    with pytest.raises(ValueError):
+        libsvx.run_segment(t, model.zero_segment(12, 12)[:20])
+    with pytest.raises(ValueError):
+        libsvx.run_segment(t, model.zero_segment(11, 11))      # L does not match the slice
+    big = torch.zeros(1 << 16, dtype=torch.complex128, device="cuda")
+    with pytest.raises(RuntimeError):
+        libsvx.run_segment(big, model.zero_segment(16, 14))   # c128 tile of 2^14 exceeds LDS
+
+
+def test_engine_reference_abi(gpu):
+    """svx_engine(st, ins_off): instructions stored after the slice in the same buffer
+    (kernels/sv/sv_engine.cpp:290-302, host/sv/runtime.py:218-233)."""
+    torch = _torch()
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(4)
+    L = 12
+    seg = random_segment(rng, L, 10, n_passes=2)
+    s0 = _rand_state(rng, L, np.complex64)
+    ins_off = (1 << L) * 8 // 32
+    buf = np.zeros((1 << L) * 8 + seg.nbytes + 64, dtype=np.uint8)
+    buf[:(1 << L) * 8] = s0.view(np.uint8)
+    buf[(1 << L) * 8:(1 << L) * 8 + seg.nbytes] = seg.view(np.uint8)
+    d = torch.from_numpy(buf).to("cuda")
+    libsvx.engine(d, ins_off)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()[:(1 << L) * 8].view(np.complex64)
+    ref = model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128)
+    _check_c64(ref, got, "engine")
+
+
+def test_prepared_replay_is_identical(gpu):
+    torch = _torch()
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(9)
+    seg = random_segment(rng, 16, 13, n_passes=5, max_m=7)
+    s0 = _rand_state(rng, 16, np.complex64)
+    h = libsvx.prepare(seg, libsvx.C64, 16)
+    outs = []
+    for _ in range(2):
+        t = torch.from_numpy(s0.copy()).to("cuda")
+        libsvx.run_prepared(t, h)
+        torch.cuda.synchronize()
+        outs.append(t.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    npass, nlaunch = libsvx.prepared_info(h)
+    assert npass == 5 and nlaunch <= 5

@@ -1,0 +1,131 @@
+"""Permutation kernel (bit-exact vs np.transpose, host/sv/model.py:74-78), the sv_swap contract
+(kernels/sv/sv_swap.cpp:10), state initialisation, FP64 reductions and the judge metrics."""
+import numpy as np
+import pytest
+
+from oracle import judge, model
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_state(rng, n, dtype):
+    s = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return (s / np.linalg.norm(s)).astype(dtype)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 7, 12, 13, 16, 21])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_permute_bit_exact(gpu, n, dt):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(n)
+    s = _rand_state(rng, n, dt)
+    src = torch.from_numpy(s).to("cuda")
+    dst = torch.empty_like(src)
+    perms = [list(range(n)), list(range(n))[::-1]] + [[int(x) for x in rng.permutation(n)] for _ in range(3)]
+    for perm in perms:
+        libsvx.permute(dst, src, perm)
+        torch.cuda.synchronize()
+        assert np.array_equal(dst.cpu().numpy(), model.remap(s, n, perm)), perm
+
+
+def test_permute_rejects_bad_input(gpu):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    a = torch.zeros(1 << 5, dtype=torch.complex64, device="cuda")
+    b = torch.zeros_like(a)
+    with pytest.raises(ValueError):
+        libsvx.permute(b, a, [0, 1, 2, 3, 3])
+    with pytest.raises(ValueError):
+        libsvx.permute(a, a, [0, 1, 2, 3, 4])
+
+
+def test_swap_words_matches_byte_swap(gpu):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(2)
+    A = rng.integers(0, 255, size=64 * 100, dtype=np.uint8)
+    B = rng.integers(0, 255, size=64 * 80, dtype=np.uint8)
+    da, db = torch.from_numpy(A).to("cuda"), torch.from_numpy(B).to("cuda")
+    libsvx.swap_words(da, db, 3, 10, 50)
+    torch.cuda.synchronize()
+    ea, eb = A.copy(), B.copy()
+    ea[3 * 64:53 * 64], eb[10 * 64:60 * 64] = B[10 * 64:60 * 64], A[3 * 64:53 * 64]
+    assert np.array_equal(da.cpu().numpy(), ea) and np.array_equal(db.cpu().numpy(), eb)
+
+
+@pytest.mark.parametrize("dt", ["c64", "c128"])
+def test_zero_and_product_state(gpu, dt):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    tdt = torch.complex64 if dt == "c64" else torch.complex128
+    s = torch.full((1 << 11,), 3 + 1j, dtype=tdt, device="cuda")
+    libsvx.zero_state(s)
+    torch.cuda.synchronize()
+    h = s.cpu().numpy()
+    assert h[0] == 1 and not h[1:].any()
+    rng = np.random.default_rng(5)
+    n = 11
+    th, ph = rng.uniform(0, np.pi, n), rng.uniform(0, 2 * np.pi, n)
+    amps = np.stack([np.cos(th / 2), np.exp(1j * ph) * np.sin(th / 2)], axis=1)
+    libsvx.product_state(s, amps)
+    torch.cuda.synchronize()
+    ref = np.ones(1, dtype=np.complex128)
+    for q in range(n):
+        ref = np.kron(amps[q], ref)
+    tol = 1e-6 if dt == "c64" else 1e-14
+    assert np.max(np.abs(s.cpu().numpy() - ref)) < tol
+
+
+@pytest.mark.parametrize("n", [3, 8, 13, 16, 20])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_norm_and_marginals_fp64(gpu, n, dt):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(n + 7)
+    s = _rand_state(rng, n, dt)
+    t = torch.from_numpy(s).to("cuda")
+    p = np.abs(s.astype(np.complex128)) ** 2
+    for fm in ([], list(range(min(n, 8))), [int(x) for x in rng.permutation(n)[:min(n, 8)]],
+               list(range(n - min(n, 3), n))):
+        norm, marg = libsvx.reduce_f64(t, fm)
+        assert abs(norm - p.sum()) < 1e-12
+        if fm:
+            ref = np.zeros(1 << len(fm))
+            idx = np.arange(1 << n)
+            x = np.zeros_like(idx)
+            for i, b in enumerate(fm):
+                x |= ((idx >> b) & 1) << i
+            np.add.at(ref, x, p)
+            assert np.max(np.abs(marg - ref)) < 1e-13, fm
+            n2, m2 = libsvx.reduce_f64(t, fm)
+            assert n2 == norm and np.array_equal(m2, marg)     # fixed reduction order
+
+
+@pytest.mark.parametrize("n", [4, 12, 18])
+def test_fidelity_metrics_vs_oracle(gpu, n):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(n)
+    a = _rand_state(rng, n, np.complex128)
+    b = (a * np.exp(0.3j) + 1e-4 * _rand_state(rng, n, np.complex128)).astype(np.complex64)
+    r = libsvx.fidelity_f64(torch.from_numpy(a).to("cuda"), torch.from_numpy(b).to("cuda"))
+    o = judge.metrics(a, b)
+    assert abs(r["fidelity"] - o["fidelity"]) < 1e-12
+    assert abs(r["one_minus_f"] - o["one_minus_f"]) < 1e-12 * max(1.0, o["one_minus_f"]) + 1e-15
+    assert abs(r["max_abs_err"] - o["max_abs_err"]) < 1e-9
+
+
+def test_convert_roundtrip(gpu):
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    rng = np.random.default_rng(1)
+    a = _rand_state(rng, 10, np.complex64)
+    t64 = torch.from_numpy(a).to("cuda")
+    t128 = torch.empty(a.size, dtype=torch.complex128, device="cuda")
+    libsvx.convert(t128, t64)
+    back = torch.empty_like(t64)
+    libsvx.convert(back, t128)
+    torch.cuda.synchronize()
+    assert np.array_equal(t128.cpu().numpy(), a.astype(np.complex128))
+    assert np.array_equal(back.cpu().numpy(), a)

@@ -1,0 +1,89 @@
+"""End-to-end runs through SVRuntime (the drop-in for host/sv/runtime.py:SVRuntime.run): compiled
+circuits on the GPU against the gate-level complex128 simulator (n <= 12), the golden model
+(20 qubits), sampling (tests/test_runtime_mock.py:33-42: 20000 shots, TVD < 0.12), and the
+30-qubit workload at full size through size-independent properties (norm, c64 vs c128 fidelity)."""
+import numpy as np
+import pytest
+
+from oracle import dense, judge, model
+from u55c_statevector_30q_amd import circuit as C
+from u55c_statevector_30q_amd import compiler as K
+
+pytestmark = pytest.mark.gpu
+
+FAMS = ["ghz", "qft", "rqc", "vqe", "grover"]
+
+
+@pytest.mark.parametrize("target", ["mi355x", "reference"])
+def test_families_vs_dense(gpu, target):
+    from u55c_statevector_30q_amd.runtime import SVRuntime
+    rt = SVRuntime(target=target)
+    worst = 0.0
+    for n in (3, 5, 9, 12):
+        for fam in FAMS:
+            if fam == "grover" and n > 9:
+                continue
+            qc = C.make(fam, n)
+            ref = dense.simulate(qc.tuples(), n)
+            for k in ((None, 5) if target == "reference" else (None, 6)):
+                out = rt.run(qc, shots=16, k=k, seed=1)
+                f = judge.fidelity(ref, out["state"])
+                worst = max(worst, 1 - f)
+                assert 1 - f < 1e-5, (target, fam, n, k, out["stats"])
+    assert worst < 1e-5
+
+
+def test_c128_runtime_matches_dense_tightly(gpu):
+    from u55c_statevector_30q_amd.runtime import SVRuntime
+    rt = SVRuntime(dtype="c128")
+    for qc in (C.make("qft", 10), C.u3_cx_layers(11, 8, 3), C.cp_ladder(9)):
+        ref = dense.simulate(qc.tuples(), qc.num_qubits)
+        out = rt.run(qc, shots=0)
+        assert judge.metrics(ref, out["state"])["max_abs_err"] < 1e-12
+
+
+def test_sampling_tvd(gpu):
+    from u55c_statevector_30q_amd.runtime import SVRuntime
+    rt = SVRuntime()
+    qc = C.make("rqc", 10)
+    r = rt.run(qc, shots=20000, seed=3)
+    p = np.abs(dense.simulate(qc.tuples(), 10)) ** 2
+    emp = np.zeros_like(p)
+    for i, c in r["counts"].items():
+        emp[i] = c / 20000
+    assert 0.5 * np.abs(emp - p).sum() < 0.12
+
+
+def test_cpu20_workload_vs_golden_model(gpu):
+    """The 20-qubit CPU workload: GPU c64 (both stream targets) vs the c128 golden model."""
+    from u55c_statevector_30q_amd.runtime import SVRuntime
+    qc = C.workload_cpu20()
+    prog, fmap, st = K.compile_gpu(qc)
+    ref = model.run_program(prog, 20, 1, fmap, ctype=np.complex128)
+    for target in ("mi355x", "reference"):
+        out = SVRuntime(target=target).run(qc, shots=0)
+        m = judge.metrics(ref, out["state"])
+        assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 1e-5, (target, m)
+
+
+def test_gpu30_workload_full_size(gpu):
+    """30 qubits, c64 (8 GiB) and c128 (16 GiB) runs of the same records: norm within 1e-5 and
+    fidelity >= 1 - 1e-6 (FP64 judge on the device)."""
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    qc = C.workload_gpu30()
+    prog, fmap, st = K.compile_gpu(qc)
+    assert all(kind == "segment" for kind, _ in prog)
+    s64 = torch.empty(1 << 30, dtype=torch.complex64, device="cuda")
+    s128 = torch.empty(1 << 30, dtype=torch.complex128, device="cuda")
+    for s in (s64, s128):
+        libsvx.zero_state(s)
+        for kind, w in prog:
+            libsvx.run_segment(s, w)
+    torch.cuda.synchronize()
+    norm, _ = libsvx.reduce_f64(s64)
+    assert abs(norm - 1) < 1e-5
+    r = libsvx.fidelity_f64(s128, s64)
+    assert r["one_minus_f"] < 1e-6 and r["fidelity"] > 1 - 1e-6, r
+    del s64, s128
+    torch.cuda.empty_cache()

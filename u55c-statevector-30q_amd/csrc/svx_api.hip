@@ -1,0 +1,825 @@
+// libsvx C ABI (include/svx.h): stream validation, pass/group/op descriptor construction,
+// device-resident descriptor caches, launches.
+//
+// A QSV2 segment (header [MAGIC, L, n_passes, k, 0,0,0,0]; per pass [m, h0, h1, n_ops, h2, h3, 0, 0];
+// per op [OP_MUX2, t, cmask, 0 x 5, M0 8 x f32, M1 8 x f32], host/sv/isa.py:5-24) is parsed once
+// into per-pass descriptors.  Each pass's ops are split into groups: maximal runs of consecutive
+// ops whose targets span at most 5 tile bits (the register bits of the pass kernel); every op is
+// classified (unconditional / controlled with M0 = I / general multiplexed; full / diagonal /
+// phase / exact X) by exact comparison of its matrices, which only removes multiplications by
+// exact zeros and ones.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/svx.h"
+#include "svx_internal.h"
+
+using namespace svx;
+
+namespace {
+
+constexpr uint32_t kMagic = 0x51535632u;
+constexpr int kHdr = 8, kPassW = 8, kOpW = 24, kOpMux2 = 1, kOpHighX = 0x48;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+int hipfail(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return SVX_E_HIP;
+}
+#define HIPCHK(x)                                    \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) return hipfail(e_, #x); \
+    } while (0)
+
+struct DeviceState {
+    bool ready = false;
+    svx_caps caps{};
+    // library scratch for reductions and product-state tables
+    double *part = nullptr;
+    size_t part_bytes = 0;
+    void *misc = nullptr;          // small descriptors / outputs
+    size_t misc_bytes = 0;
+    std::mutex mu;
+};
+std::mutex g_mu;
+std::map<int, std::unique_ptr<DeviceState>> g_dev;
+
+int current_device(int *dev) {
+    HIPCHK(hipGetDevice(dev));
+    return 0;
+}
+
+int ensure_device(int dev, DeviceState **out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto &slot = g_dev[dev];
+    if (!slot) slot.reset(new DeviceState());
+    DeviceState *ds = slot.get();
+    if (!ds->ready) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(SVX_E_NODEV, std::string("libsvx is built for gfx950, device is ") + prop.gcnArchName);
+        HIPCHK(set_kernel_attributes());
+        ds->caps.device = dev;
+        ds->caps.cu_count = prop.multiProcessorCount;
+        ds->caps.lds_per_block = (int64_t)prop.sharedMemPerBlock;
+        ds->caps.hbm_bytes = (int64_t)prop.totalGlobalMem;
+        ds->caps.wavefront = prop.warpSize;
+        ds->caps.max_k_c64 = kMaxKc64;
+        ds->caps.max_k_c128 = kMaxKc128;
+        ds->caps.max_high = kMaxM;
+        std::snprintf(ds->caps.arch, sizeof(ds->caps.arch), "%s", prop.gcnArchName);
+        ds->ready = true;
+    }
+    *out = ds;
+    return 0;
+}
+
+int get_ds(DeviceState **ds) {
+    int dev = 0;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    return ensure_device(dev, ds);
+}
+
+inline int ilog2_exact(int64_t x) {
+    if (x <= 0 || (x & (x - 1))) return -1;
+    int b = 0;
+    while ((int64_t(1) << b) < x) ++b;
+    return b;
+}
+
+// ------------------------------------------------------------------------------------------ parse
+struct RawOp {
+    int t;
+    uint32_t cmask;
+    float f[16];
+};
+struct RawPass {
+    int m;
+    int h[kMaxM];
+    std::vector<RawOp> ops;
+};
+struct RawSeg {
+    int L, k;
+    std::vector<RawPass> passes;
+};
+
+int parse_segment(const int32_t *w, int64_t nw, RawSeg *out) {
+    if (nw < kHdr) return fail(SVX_E_FORMAT, "QSV2 stream shorter than its 8-word header");
+    if ((uint32_t)w[0] != kMagic) return fail(SVX_E_MAGIC, "bad QSV2 magic");
+    const int L = w[1], npass = w[2], k = w[3];
+    if (L < 1 || L > 40) return fail(SVX_E_FORMAT, "QSV2 header: L out of range");
+    if (k < 1 || k > L) return fail(SVX_E_FORMAT, "QSV2 header: tile bits k must satisfy 1 <= k <= L");
+    if (npass < 0) return fail(SVX_E_FORMAT, "QSV2 header: negative pass count");
+    out->L = L;
+    out->k = k;
+    out->passes.clear();
+    int64_t pos = kHdr;
+    for (int p = 0; p < npass; ++p) {
+        if (pos + kPassW > nw) return fail(SVX_E_FORMAT, "QSV2 stream truncated in a pass header");
+        RawPass rp;
+        rp.m = w[pos];
+        const int nops = w[pos + 3];
+        int hv[4] = {w[pos + 1], w[pos + 2], w[pos + 4], w[pos + 5]};
+        pos += kPassW;
+        if (rp.m < 0 || rp.m > kMaxM || rp.m > k)
+            return fail(SVX_E_FORMAT, "pass has m = " + std::to_string(rp.m) + " high qubits (max " +
+                                          std::to_string(std::min(k, kMaxM)) + ")");
+        for (int i = 0; i < std::min(rp.m, 4); ++i) rp.h[i] = hv[i];
+        if (rp.m > 4) {
+            if (pos + kPassW > nw || w[pos] != kOpHighX)
+                return fail(SVX_E_FORMAT, "pass with m > 4 lacks its high-qubit record");
+            for (int i = 4; i < rp.m; ++i) rp.h[i] = w[pos + 1 + (i - 4)];
+            pos += kPassW;
+        }
+        const int lowb = k - rp.m;
+        for (int i = 0; i < rp.m; ++i) {
+            if (rp.h[i] < lowb || rp.h[i] >= L)
+                return fail(SVX_E_FORMAT, "high qubit outside [k-m, L)");
+            for (int j = 0; j < i; ++j)
+                if (rp.h[j] == rp.h[i]) return fail(SVX_E_FORMAT, "repeated high qubit in a pass");
+        }
+        if (nops < 0) return fail(SVX_E_FORMAT, "negative op count");
+        if (pos + (int64_t)nops * kOpW > nw) return fail(SVX_E_FORMAT, "QSV2 stream truncated in an op record");
+        rp.ops.resize(nops);
+        for (int o = 0; o < nops; ++o) {
+            const int32_t *r = w + pos;
+            if (r[0] != kOpMux2) return fail(SVX_E_FORMAT, "unknown opcode " + std::to_string(r[0]));
+            if (r[1] < 0 || r[1] >= k) return fail(SVX_E_FORMAT, "op target outside the tile");
+            if ((uint32_t)r[2] >> k) return fail(SVX_E_FORMAT, "op control mask outside the tile");
+            rp.ops[o].t = r[1];
+            rp.ops[o].cmask = (uint32_t)r[2];
+            std::memcpy(rp.ops[o].f, r + 8, 16 * sizeof(float));
+            pos += kOpW;
+        }
+        out->passes.push_back(std::move(rp));
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------ build
+template <typename T>
+bool is_id(const T *m) { return m[0] == T(1) && m[1] == 0 && m[2] == 0 && m[3] == 0 && m[4] == 0 && m[5] == 0 && m[6] == T(1) && m[7] == 0; }
+template <typename T>
+bool is_x(const T *m) { return m[0] == 0 && m[1] == 0 && m[2] == T(1) && m[3] == 0 && m[4] == T(1) && m[5] == 0 && m[6] == 0 && m[7] == 0; }
+template <typename T>
+bool is_diag(const T *m) { return m[2] == 0 && m[3] == 0 && m[4] == 0 && m[5] == 0; }
+template <typename T>
+bool same(const T *a, const T *b) {
+    for (int e = 0; e < 8; ++e)
+        if (!(a[e] == b[e])) return false;
+    return true;
+}
+
+struct LaunchInfo {
+    bool small;
+    int K;
+    uint32_t grid;
+    size_t lds;
+    int64_t pass_index;   // index into the PassDev array
+    bool empty;           // no op changes the state (skip launch)
+};
+
+struct Prepared {
+    int dtype = 0, nbits = 0;
+    std::vector<int32_t> words;
+    void *dev = nullptr;        // [PassDev x P][GroupDev x G][OpDev<T> x O]
+    size_t pass_off = 0, group_off = 0, op_off = 0;
+    std::vector<LaunchInfo> launches;
+    int device = 0;
+};
+
+std::mutex g_prep_mu;
+std::vector<std::unique_ptr<Prepared>> g_prepared;              // handle = index + 1
+std::multimap<uint64_t, int64_t> g_prep_index;                  // hash -> handle
+size_t g_prep_bytes = 0;
+
+uint64_t fnv1a(const int32_t *w, int64_t n, uint64_t seed) {
+    uint64_t h = 1469598103934665603ull ^ seed;
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(w);
+    for (int64_t i = 0; i < n * 4; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+template <typename T>
+int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
+    const int k = seg.k;
+    const int maxK = dtype == SVX_C64 ? kMaxKc64 : kMaxKc128;
+    if (k > maxK)
+        return fail(SVX_E_CAPACITY, "tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS tile limit 2^" +
+                                        std::to_string(maxK));
+    const bool small = nbits <= kSmallMaxBits;
+    int K = 0;
+    if (!small) {
+        K = std::max(k, kMinK);
+        if (K - k > kMaxTileSub) K = k + kMaxTileSub;
+        if (K < kMinK) K = kMinK;   // unreachable with kMaxTileSub >= kMinK
+        if (K > nbits) K = nbits;
+        if (K > maxK) K = maxK;
+        if (K < k) return fail(SVX_E_CAPACITY, "tile larger than the state");
+    }
+    std::vector<PassDev> passes;
+    std::vector<GroupDev> groups;
+    std::vector<OpDev<T>> ops;
+    for (const RawPass &rp : seg.passes) {
+        PassDev pd{};
+        pd.k = k;
+        pd.m = rp.m;
+        pd.K = small ? nbits : K;
+        pd.nbits = nbits;
+        for (int i = 0; i < rp.m; ++i) pd.h[i] = pd.hs[i] = rp.h[i];
+        std::sort(pd.hs, pd.hs + rp.m);
+        pd.group0 = (int)groups.size();
+        pd.op0 = (int)ops.size();
+        // classify
+        std::vector<OpDev<T>> pops;
+        for (const RawOp &ro : rp.ops) {
+            OpDev<T> od{};
+            for (int e = 0; e < 8; ++e) {
+                od.m0[e] = T(ro.f[e]);
+                od.m1[e] = T(ro.f[8 + e]);
+            }
+            od.t = ro.t;
+            od.cmask = (int32_t)ro.cmask;
+            const bool unc = ro.cmask == 0 || same(od.m0, od.m1);
+            if (small) {
+                od.kind = K_MUX;
+                pops.push_back(od);
+                continue;
+            }
+            if (unc) {
+                if (is_id(od.m0)) continue;   // exact no-op
+                od.kind = is_x(od.m0) ? K_XSWAP : (is_diag(od.m0) ? K_DIAG : K_FULL);
+            } else if (is_id(od.m0)) {
+                if (is_x(od.m1)) od.kind = K_CXSWAP;
+                else if (is_diag(od.m1) && od.m1[0] == T(1) && od.m1[1] == T(0)) od.kind = K_CPHASE;
+                else if (is_diag(od.m1)) od.kind = K_CDIAG;
+                else od.kind = K_CFULL;
+            } else {
+                od.kind = K_MUX;
+            }
+            pops.push_back(od);
+        }
+        pd.nops = (int)pops.size();
+        if (small) {
+            for (auto &o : pops) ops.push_back(o);
+            pd.ngroups = 0;
+            passes.push_back(pd);
+            continue;
+        }
+        // groups: maximal runs of ops whose targets fit kR register bits
+        size_t i = 0;
+        while (i < pops.size()) {
+            std::vector<int> tg;
+            size_t j = i;
+            for (; j < pops.size(); ++j) {
+                const int t = pops[j].t;
+                if (std::find(tg.begin(), tg.end(), t) == tg.end()) {
+                    if ((int)tg.size() == kR) break;
+                    tg.push_back(t);
+                }
+            }
+            // register bits: the targets, then filler bits from the top of the workgroup tile
+            std::vector<int> rb = tg;
+            for (int b = K - 1; b >= 0 && (int)rb.size() < kR; --b)
+                if (std::find(rb.begin(), rb.end(), b) == rb.end()) rb.push_back(b);
+            GroupDev gd{};
+            gd.nops = (int)(j - i);
+            gd.op0 = (int)ops.size();
+            uint32_t regmask = 0;
+            for (int q = 0; q < kR; ++q) {
+                gd.b[q] = gd.bs[q] = rb[q];
+                regmask |= 1u << rb[q];
+            }
+            std::sort(gd.bs, gd.bs + kR);
+            for (size_t o = i; o < j; ++o) {
+                OpDev<T> od = pops[o];
+                od.tr = (int)(std::find(rb.begin(), rb.end(), od.t) - rb.begin());
+                uint32_t cmr = 0;
+                for (int q = 0; q < kR; ++q)
+                    if ((uint32_t)od.cmask >> rb[q] & 1u) cmr |= 1u << q;
+                od.cmr = cmr;
+                od.cmt = (uint32_t)od.cmask & ~regmask;
+                if (od.cmask == 0 || od.kind <= K_XSWAP) {
+                    od.cmr = 0;
+                    od.cmt = 0;
+                }
+                ops.push_back(od);
+            }
+            groups.push_back(gd);
+            i = j;
+        }
+        pd.ngroups = (int)groups.size() - pd.group0;
+        passes.push_back(pd);
+    }
+    // layout: passes, groups, ops (256-byte aligned sections)
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    pr->pass_off = 0;
+    pr->group_off = al(passes.size() * sizeof(PassDev));
+    pr->op_off = pr->group_off + al(groups.size() * sizeof(GroupDev));
+    const size_t total = std::max<size_t>(256, pr->op_off + al(ops.size() * sizeof(OpDev<T>)));
+    std::vector<unsigned char> host(total, 0);
+    if (!passes.empty()) std::memcpy(host.data() + pr->pass_off, passes.data(), passes.size() * sizeof(PassDev));
+    if (!groups.empty()) std::memcpy(host.data() + pr->group_off, groups.data(), groups.size() * sizeof(GroupDev));
+    if (!ops.empty()) std::memcpy(host.data() + pr->op_off, ops.data(), ops.size() * sizeof(OpDev<T>));
+    HIPCHK(hipMalloc(&pr->dev, total));
+    HIPCHK(hipMemcpy(pr->dev, host.data(), total, hipMemcpyHostToDevice));
+    g_prep_bytes += total;
+    pr->launches.clear();
+    for (size_t p = 0; p < passes.size(); ++p) {
+        LaunchInfo li{};
+        li.small = small;
+        li.K = passes[p].K;
+        li.pass_index = (int64_t)p;
+        li.empty = passes[p].nops == 0;
+        if (small) {
+            li.grid = 1;
+            li.lds = 0;
+        } else {
+            li.grid = (uint32_t)(uint64_t(1) << (nbits - K));
+            li.lds = pass_lds_bytes(K, passes[p].m, k, (int)sizeof(T) * 2, false, nbits);
+        }
+        pr->launches.push_back(li);
+    }
+    return 0;
+}
+
+int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle, Prepared **out) {
+    if (!w || nw <= 0) return fail(SVX_E_ARG, "empty QSV2 stream");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "dtype must be SVX_C64 or SVX_C128");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
+                                        (uint64_t)ds->caps.device * 97u);
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    auto range = g_prep_index.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it) {
+        Prepared *p = g_prepared[it->second - 1].get();
+        if (p && p->dtype == dtype && p->nbits == nbits && p->device == ds->caps.device &&
+            (int64_t)p->words.size() == nw && std::memcmp(p->words.data(), w, nw * 4) == 0) {
+            *handle = it->second;
+            if (out) *out = p;
+            return 0;
+        }
+    }
+    RawSeg seg;
+    rc = parse_segment(w, nw, &seg);
+    if (rc) return rc;
+    if (seg.L > nbits) return fail(SVX_E_ARG, "segment L exceeds the state size");
+    std::unique_ptr<Prepared> pr(new Prepared());
+    pr->dtype = dtype;
+    pr->nbits = nbits;
+    pr->device = ds->caps.device;
+    pr->words.assign(w, w + nw);
+    rc = dtype == SVX_C64 ? build_prepared<float>(seg, nbits, dtype, pr.get())
+                          : build_prepared<double>(seg, nbits, dtype, pr.get());
+    if (rc) return rc;
+    g_prepared.push_back(std::move(pr));
+    const int64_t hd = (int64_t)g_prepared.size();
+    g_prep_index.insert({h, hd});
+    *handle = hd;
+    if (out) *out = g_prepared.back().get();
+    return 0;
+}
+
+int run_prepared(void *state, Prepared *pr, hipStream_t s) {
+    if (!state) return fail(SVX_E_ARG, "null state pointer");
+    if (reinterpret_cast<uintptr_t>(state) & 15) return fail(SVX_E_ARG, "state must be 16-byte aligned");
+    unsigned char *base = static_cast<unsigned char *>(pr->dev);
+    const PassDev *pd = reinterpret_cast<const PassDev *>(base + pr->pass_off);
+    const GroupDev *gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
+    for (const LaunchInfo &li : pr->launches) {
+        if (li.empty) continue;
+        hipError_t e;
+        if (pr->dtype == SVX_C64)
+            e = launch_pass_c64(li.K, li.small, state, pd + li.pass_index, gd,
+                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.grid, li.lds, s);
+        else
+            e = launch_pass_c128(li.K, li.small, state, pd + li.pass_index, gd,
+                                 reinterpret_cast<const OpDev<double> *>(base + pr->op_off), li.grid, li.lds, s);
+        if (e != hipSuccess) return hipfail(e, "pass kernel launch");
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------ permute
+std::mutex g_perm_mu;
+std::map<std::vector<int32_t>, void *> g_perm_cache;   // key: [device, n, perm...] -> PermDev
+
+int build_perm(int n, const int32_t *perm, PermDev *pd, uint32_t *grid) {
+    std::vector<int> inv(n, -1);
+    for (int b = 0; b < n; ++b) {
+        if (perm[b] < 0 || perm[b] >= n || inv[perm[b]] != -1) return fail(SVX_E_ARG, "perm is not a permutation of 0..n-1");
+        inv[perm[b]] = b;
+    }
+    std::memset(pd, 0, sizeof(*pd));
+    int c, v;
+    if (n <= 12) {
+        c = std::min(6, n);
+        v = n;
+    } else {
+        c = 6;
+        v = 12;
+    }
+    std::vector<char> inV(n, 0);
+    for (int b = 0; b < c; ++b) inV[b] = 1;
+    for (int o = 0; o < c; ++o) inV[inv[o]] = 1;
+    int cnt = 0;
+    for (int b = 0; b < n; ++b) cnt += inV[b];
+    for (int b = 0; b < n && cnt < v; ++b)
+        if (!inV[b]) {
+            inV[b] = 1;
+            ++cnt;
+        }
+    std::vector<int> V, F;
+    for (int b = 0; b < n; ++b) (inV[b] ? V : F).push_back(b);
+    if ((int)V.size() != v) return fail(SVX_E_ARG, "internal: permutation tile size");
+    std::vector<int> rank(n, -1);
+    for (int i = 0; i < v; ++i) rank[V[i]] = i;
+    std::vector<int> OV;   // old positions of the tile bits, ascending
+    for (int i = 0; i < v; ++i) OV.push_back(perm[V[i]]);
+    std::sort(OV.begin(), OV.end());
+    for (int p = 0; p < c; ++p)
+        if (OV[p] != p) return fail(SVX_E_ARG, "internal: permutation low bits");
+    pd->n = n;
+    pd->c = c;
+    pd->v = v;
+    pd->nfix = (int)F.size();
+    for (size_t i = 0; i < F.size(); ++i) {
+        pd->fixnew[i] = F[i];
+        pd->fixold[i] = perm[F[i]];
+    }
+    // read-order index q: bit p <-> old bit OV[p] -> new bit inv[OV[p]] -> local bit rank[...]
+    auto rdpos = [&](uint32_t q) {
+        uint32_t pos = 0;
+        for (int p = 0; p < v; ++p)
+            if ((q >> p) & 1u) pos |= 1u << rank[inv[OV[p]]];
+        return pos;
+    };
+    for (uint32_t x = 0; x < 64; ++x) {
+        pd->rdlo[x] = (uint16_t)(x < (1u << std::min(v, 6)) ? rdpos(x) : 0);
+        pd->rdhi[x] = (uint16_t)((v > 6 && x < (1u << (v - 6))) ? rdpos(x << 6) : 0);
+        uint64_t oh = 0, nh = 0;
+        if (x < (1u << (v - c))) {
+            for (int p = c; p < v; ++p)
+                if ((x >> (p - c)) & 1u) {
+                    oh |= uint64_t(1) << OV[p];
+                    nh |= uint64_t(1) << V[p];
+                }
+        }
+        pd->oldhi[x] = oh;
+        pd->newhi[x] = nh;
+    }
+    *grid = (uint32_t)(uint64_t(1) << (n - v));
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================================ C ABI
+extern "C" {
+
+const char *svx_last_error(void) { return g_err.c_str(); }
+const char *svx_version(void) { return "libsvx 0.1 (gfx950)"; }
+
+int svx_init(int32_t device, svx_caps *out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(SVX_E_NODEV, "no HIP device is visible");
+    if (device < 0 || device >= n) return fail(SVX_E_NODEV, "no HIP device " + std::to_string(device));
+    DeviceState *ds = nullptr;
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    int rc = ensure_device(device, &ds);
+    if (cur != device) (void)hipSetDevice(cur);
+    if (rc) return rc;
+    if (out) *out = ds->caps;
+    return 0;
+}
+
+int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle) {
+    if (!handle) return fail(SVX_E_ARG, "null handle pointer");
+    if (state_bits < 1 || state_bits > 40) return fail(SVX_E_ARG, "state_bits out of range");
+    return prepare(qsv2, n_words, dtype, state_bits, handle, nullptr);
+}
+
+int svx_run_prepared(void *state, int64_t handle, void *stream) {
+    Prepared *pr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_prep_mu);
+        if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
+            return fail(SVX_E_ARG, "unknown prepared-segment handle");
+        pr = g_prepared[handle - 1].get();
+    }
+    return run_prepared(state, pr, (hipStream_t)stream);
+}
+
+int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches) {
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
+        return fail(SVX_E_ARG, "unknown prepared-segment handle");
+    Prepared *pr = g_prepared[handle - 1].get();
+    int nl = 0;
+    for (auto &li : pr->launches) nl += li.empty ? 0 : 1;
+    if (n_passes) *n_passes = (int32_t)pr->launches.size();
+    if (n_launches) *n_launches = nl;
+    return 0;
+}
+
+int svx_run_segment(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t dtype, const int32_t *qsv2,
+                    int64_t n_words, void *stream) {
+    const int L = ilog2_exact(n_slice_amps);
+    if (L < 1) return fail(SVX_E_ARG, "n_slice_amps must be a power of two >= 2");
+    if (n_slices < 1) return fail(SVX_E_ARG, "n_slices must be >= 1");
+    if (!qsv2 || n_words < kHdr) return fail(SVX_E_FORMAT, "QSV2 stream shorter than its 8-word header");
+    if ((uint32_t)qsv2[0] != kMagic) return fail(SVX_E_MAGIC, "bad QSV2 magic");
+    if (qsv2[1] != L)
+        return fail(SVX_E_ARG, "segment L = " + std::to_string(qsv2[1]) + " does not match the slice of 2^" + std::to_string(L));
+    const int pb = ilog2_exact(n_slices);
+    if (pb >= 0) {
+        Prepared *pr = nullptr;
+        int64_t h = 0;
+        int rc = prepare(qsv2, n_words, dtype, L + pb, &h, &pr);
+        if (rc) return rc;
+        return run_prepared(state, pr, (hipStream_t)stream);
+    }
+    // a slice count that is not a power of two: one launch sequence per slice
+    Prepared *pr = nullptr;
+    int64_t h = 0;
+    int rc = prepare(qsv2, n_words, dtype, L, &h, &pr);
+    if (rc) return rc;
+    const size_t es = dtype == SVX_C64 ? 8 : 16;
+    for (int32_t p = 0; p < n_slices; ++p) {
+        rc = run_prepared(static_cast<unsigned char *>(state) + (size_t)p * n_slice_amps * es, pr, (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int svx_engine(void *st, int64_t ins_off, void *stream) {
+    if (!st || ins_off < 0) return fail(SVX_E_ARG, "bad buffer or instruction offset");
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned char *ins = static_cast<const unsigned char *>(st) + ins_off * 32;
+    int32_t hdr[kHdr];
+    HIPCHK(hipMemcpyAsync(hdr, ins, sizeof(hdr), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if ((uint32_t)hdr[0] != kMagic) return fail(SVX_E_MAGIC, "bad QSV2 magic");
+    // walk pass headers to find the stream length
+    std::vector<int32_t> words(hdr, hdr + kHdr);
+    int64_t pos = kHdr;
+    for (int p = 0; p < hdr[2]; ++p) {
+        int32_t ph[kPassW];
+        HIPCHK(hipMemcpy(ph, ins + pos * 4, sizeof(ph), hipMemcpyDeviceToHost));
+        int64_t len = kPassW + (int64_t)std::max(0, ph[3]) * kOpW + (ph[0] > 4 ? kPassW : 0);
+        if (len > (int64_t(4) << 20)) return fail(SVX_E_FORMAT, "pass longer than the 4 MiB instruction region");
+        const size_t old = words.size();
+        words.resize(old + len);
+        HIPCHK(hipMemcpy(words.data() + old, ins + pos * 4, len * 4, hipMemcpyDeviceToHost));
+        pos += len;
+    }
+    const int L = hdr[1];
+    if (ins_off * 32 < (int64_t(1) << L) * 8) return fail(SVX_E_ARG, "instruction region overlaps the 2^L slice");
+    return svx_run_segment(st, int64_t(1) << L, 1, SVX_C64, words.data(), (int64_t)words.size(), stream);
+}
+
+int svx_permute(void *dst, const void *src, int32_t n, const int32_t *perm, int32_t dtype, void *stream) {
+    if (!dst || !src || !perm) return fail(SVX_E_ARG, "null pointer");
+    if (dst == src) return fail(SVX_E_ARG, "svx_permute is out of place: dst must differ from src");
+    if (n < 1 || n > 40) return fail(SVX_E_ARG, "n out of range");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "bad dtype");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    std::vector<int32_t> key{ds->caps.device, n};
+    key.insert(key.end(), perm, perm + n);
+    void *pdv = nullptr;
+    uint32_t grid = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_perm_mu);
+        PermDev pd;
+        rc = build_perm(n, perm, &pd, &grid);
+        if (rc) return rc;
+        auto it = g_perm_cache.find(key);
+        if (it != g_perm_cache.end()) {
+            pdv = it->second;
+        } else {
+            HIPCHK(hipMalloc(&pdv, sizeof(PermDev)));
+            HIPCHK(hipMemcpy(pdv, &pd, sizeof(PermDev), hipMemcpyHostToDevice));
+            g_perm_cache[key] = pdv;
+        }
+    }
+    const int v = n <= 12 ? n : 12;
+    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 5)) * (dtype == SVX_C64 ? 8 : 16);
+    hipError_t e = launch_permute(dtype, dst, src, (const PermDev *)pdv, grid, lds, (hipStream_t)stream);
+    if (e != hipSuccess) return hipfail(e, "permute launch");
+    return 0;
+}
+
+int svx_swap_words(void *a, void *b, int64_t offa, int64_t offb, int64_t nwords, void *stream) {
+    if (!a || !b || offa < 0 || offb < 0 || nwords < 0) return fail(SVX_E_ARG, "bad swap arguments");
+    const char *pa = static_cast<char *>(a) + offa * 64, *pb = static_cast<char *>(b) + offb * 64;
+    if (pa < pb + nwords * 64 && pb < pa + nwords * 64 && nwords > 0) return fail(SVX_E_ARG, "swap ranges overlap");
+    hipError_t e = launch_swap_words(a, b, offa, offb, nwords, (hipStream_t)stream);
+    if (e != hipSuccess) return hipfail(e, "swap launch");
+    return 0;
+}
+
+int svx_zero_state(void *state, int32_t n, int32_t dtype, void *stream) {
+    if (!state || n < 0 || n > 40) return fail(SVX_E_ARG, "bad zero-state arguments");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "bad dtype");
+    hipError_t e = launch_zero(dtype, state, int64_t(1) << n, (hipStream_t)stream);
+    if (e != hipSuccess) return hipfail(e, "zero launch");
+    return 0;
+}
+
+int svx_convert(void *dst, int32_t dt_dst, const void *src, int32_t dt_src, int64_t n_amps, void *stream) {
+    if (!dst || !src || n_amps < 0) return fail(SVX_E_ARG, "bad convert arguments");
+    hipError_t e = launch_convert(dst, dt_dst, src, dt_src, n_amps, (hipStream_t)stream);
+    if (e != hipSuccess) return hipfail(e, "convert launch");
+    return 0;
+}
+
+static int ensure_scratch(DeviceState *ds, size_t part_bytes, size_t misc_bytes) {
+    if (ds->part_bytes < part_bytes) {
+        if (ds->part) HIPCHK(hipFree(ds->part));
+        ds->part = nullptr;
+        HIPCHK(hipMalloc(&ds->part, part_bytes));
+        ds->part_bytes = part_bytes;
+    }
+    if (ds->misc_bytes < misc_bytes) {
+        if (ds->misc) HIPCHK(hipFree(ds->misc));
+        ds->misc = nullptr;
+        HIPCHK(hipMalloc(&ds->misc, misc_bytes));
+        ds->misc_bytes = misc_bytes;
+    }
+    return 0;
+}
+
+int svx_product_state(void *state, int32_t n, int32_t dtype, const double *amps, void *stream) {
+    if (!state || !amps || n < 1 || n > 40) return fail(SVX_E_ARG, "bad product-state arguments");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "bad dtype");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    const int nlo = n / 2, nhi = n - nlo;
+    auto table = [&](int q0, int nq) {
+        std::vector<double> t(size_t(2) << nq);
+        for (uint64_t x = 0; x < (uint64_t(1) << nq); ++x) {
+            double re = 1, im = 0;
+            for (int i = 0; i < nq; ++i) {
+                const double *a = amps + 4 * (q0 + i) + (((x >> i) & 1) ? 2 : 0);
+                const double r2 = re * a[0] - im * a[1], i2 = re * a[1] + im * a[0];
+                re = r2;
+                im = i2;
+            }
+            t[2 * x] = re;
+            t[2 * x + 1] = im;
+        }
+        return t;
+    };
+    std::vector<double> tlo = table(0, nlo), thi = table(nlo, nhi);
+    std::lock_guard<std::mutex> lk(ds->mu);
+    const size_t bytes = (tlo.size() + thi.size()) * sizeof(double) + 256;
+    rc = ensure_scratch(ds, std::max<size_t>(ds->part_bytes, 1 << 20), std::max<size_t>(bytes, 1 << 16));
+    if (rc) return rc;
+    double *dlo = static_cast<double *>(ds->misc);
+    double *dhi = dlo + tlo.size();
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(dlo, tlo.data(), tlo.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dhi, thi.data(), thi.size() * sizeof(double), hipMemcpyHostToDevice));
+    hipError_t e = launch_product(dtype, state, n, dlo, dhi, nlo, s);
+    if (e != hipSuccess) return hipfail(e, "product launch");
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int svx_reduce_f64(const void *state, int32_t n, int32_t dtype, const int32_t *fmap, int32_t n_low, double *out_norm,
+                   double *out_marginals, void *stream) {
+    if (!state || n < 1 || n > 40 || n_low < 0 || n_low > 8 || n_low > n) return fail(SVX_E_ARG, "bad reduce arguments");
+    if (n_low > 0 && (!fmap || !out_marginals)) return fail(SVX_E_ARG, "fmap and out_marginals required when n_low > 0");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "bad dtype");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    MargCfg cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    const int c = std::min(n, 13);
+    cfg.n = n;
+    cfg.c = c;
+    cfg.low8 = std::min(c, 8);
+    cfg.nlow = n_low;
+    std::vector<int> used(n, -1);
+    for (int i = 0; i < n_low; ++i) {
+        if (fmap[i] < 0 || fmap[i] >= n || used[fmap[i]] != -1) return fail(SVX_E_ARG, "fmap entries must be distinct bits < n");
+        used[fmap[i]] = i;
+    }
+    for (int b = 0; b < cfg.low8; ++b) {
+        if (used[b] >= 0) {
+            cfg.sa[cfg.nsa] = b;
+            cfg.sa_x[cfg.nsa++] = used[b];
+        } else {
+            cfg.ra[cfg.nra++] = b;
+        }
+    }
+    for (int b = cfg.low8; b < c; ++b) {
+        if (used[b] >= 0) {
+            cfg.sb[cfg.nsb] = b;
+            cfg.sb_x[cfg.nsb++] = used[b];
+        } else {
+            cfg.rb[cfg.nrb++] = b;
+        }
+    }
+    for (int b = c; b < n; ++b)
+        if (used[b] >= 0) {
+            cfg.shi[cfg.nshi] = b - c;
+            cfg.shi_x[cfg.nshi++] = used[b];
+        }
+    const int nbin = 1 << n_low;
+    const uint64_t nchunk = uint64_t(1) << (n - c);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(nchunk, 2048);
+    std::lock_guard<std::mutex> lk(ds->mu);
+    rc = ensure_scratch(ds, std::max<size_t>((size_t)grid * 256 * sizeof(double), 1 << 20), 1 << 16);
+    if (rc) return rc;
+    unsigned char *misc = static_cast<unsigned char *>(ds->misc);
+    MargCfg *cfg_dev = reinterpret_cast<MargCfg *>(misc);
+    double *out_dev = reinterpret_cast<double *>(misc + 4096);
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipMemcpy(cfg_dev, &cfg, sizeof(cfg), hipMemcpyHostToDevice));
+    hipError_t e = launch_marginals(dtype, state, cfg_dev, nbin, ds->part, grid, out_dev, s);
+    if (e != hipSuccess) return hipfail(e, "marginal launch");
+    std::vector<double> out(nbin);
+    HIPCHK(hipMemcpyAsync(out.data(), out_dev, nbin * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double norm = 0;
+    for (int x = 0; x < nbin; ++x) norm += out[x];
+    if (out_norm) *out_norm = norm;
+    if (n_low > 0) std::memcpy(out_marginals, out.data(), nbin * sizeof(double));
+    return 0;
+}
+
+int svx_fidelity_f64(const void *a, int32_t dta, const void *b, int32_t dtb, int32_t n, double *out, void *stream) {
+    if (!a || !b || !out || n < 0 || n > 40) return fail(SVX_E_ARG, "bad fidelity arguments");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    const int64_t N = int64_t(1) << n;
+    const uint32_t grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(2048, N / 1024));
+    std::lock_guard<std::mutex> lk(ds->mu);
+    rc = ensure_scratch(ds, std::max<size_t>((size_t)grid * 4 * sizeof(double), 1 << 20), 1 << 16);
+    if (rc) return rc;
+    unsigned char *misc = static_cast<unsigned char *>(ds->misc);
+    double *o4 = reinterpret_cast<double *>(misc + 8192);
+    double *coef = o4 + 8;
+    double *o2 = coef + 8;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = launch_fid_pass1(dta, a, dtb, b, N, ds->part, grid, o4, s);
+    if (e != hipSuccess) return hipfail(e, "fidelity launch");
+    double h4[4];
+    HIPCHK(hipMemcpyAsync(h4, o4, sizeof(h4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double na = std::sqrt(h4[2]), nb = std::sqrt(h4[3]);
+    if (!(na > 0) || !(nb > 0)) return fail(SVX_E_ARG, "zero-norm state in fidelity");
+    const double ipr = h4[0] / (na * nb), ipi = h4[1] / (na * nb);
+    const double ab = std::sqrt(ipr * ipr + ipi * ipi);
+    double hc[4] = {ab > 0 ? ipr / ab : 1.0, ab > 0 ? ipi / ab : 0.0, 1.0 / na, 1.0 / nb};
+    // e = b/|b| - ph a/|a| with ph = <a|b>/|<a|b>| minimises |e|
+    HIPCHK(hipMemcpy(coef, hc, sizeof(hc), hipMemcpyHostToDevice));
+    e = launch_fid_pass2(dta, a, dtb, b, N, coef, ds->part, grid, o2, s);
+    if (e != hipSuccess) return hipfail(e, "fidelity launch");
+    double h2[2];
+    HIPCHK(hipMemcpyAsync(h2, o2, sizeof(h2), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out[0] = ab * ab;
+    out[1] = h2[0] - h2[0] * h2[0] / 4;
+    out[2] = std::sqrt(h2[1]);
+    out[3] = std::sqrt(h2[0]);
+    return 0;
+}
+
+int svx_sync(void *stream) {
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Internal descriptors shared by the host side (svx_api.hip) and the kernels (svx_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svx {
+
+constexpr int kR = 5;            // register bits per group: each thread holds 2^5 amplitudes
+constexpr int kNT = 256;         // threads per workgroup of the pass kernel (4 wavefronts)
+constexpr int kMaxM = 8;         // gathered high qubits per pass (sub-block table 2^8 entries)
+constexpr int kMinK = 10;        // smallest workgroup tile of the main pass kernel
+constexpr int kMaxKc64 = 14;     // 2^14 complex64 = 128 KiB (+ padding) of LDS
+constexpr int kMaxKc128 = 13;    // 2^13 complex128 = 128 KiB (+ padding)
+constexpr int kSmallMaxBits = 9; // states of <= 2^9 amplitudes run in one workgroup (k_pass_small)
+constexpr int kMaxTileSub = 8;   // at most 2^8 QSV2 tiles per workgroup tile
+
+// op kinds (host classification of the MUX2 record; exact comparisons on the typed values)
+enum OpKind : int32_t {
+    K_FULL = 0,    // unconditional 2x2 M0 (cmask == 0 or M0 == M1)
+    K_DIAG = 1,    // unconditional diagonal M0
+    K_XSWAP = 2,   // unconditional exact X
+    K_CFULL = 3,   // controlled, M0 == I: selected pairs get M1
+    K_CDIAG = 4,   // controlled, M0 == I, M1 diagonal
+    K_CPHASE = 5,  // controlled, M0 == I, M1 = diag(1, e)
+    K_CXSWAP = 6,  // controlled, M0 == I, M1 exact X
+    K_MUX = 7,     // controlled, general: selected pairs M1, others M0
+};
+
+struct PassDev {
+    int32_t k, m, K, nbits;      // QSV2 tile bits, high count, workgroup tile bits, state bits
+    int32_t ngroups, group0, nops, op0;
+    int32_t h[kMaxM];            // physical bit of tile qubit k-m+i
+    int32_t hs[kMaxM];           // h sorted ascending (tile-id deposit)
+};
+
+struct GroupDev {
+    int32_t nops, op0;
+    int32_t b[kR];               // workgroup-tile bit held by register bit q
+    int32_t bs[kR];              // b sorted ascending (slot deposit)
+    int32_t pad_[2];
+};
+
+template <typename T>
+struct OpDev {
+    int32_t kind, tr;            // kind, target register bit
+    uint32_t cmr, cmt;           // control mask over register bits / over other tile bits
+    int32_t t, cmask;            // raw tile-coordinate target and control mask (small kernel)
+    int32_t pad_[2];
+    T m0[8], m1[8];              // (re, im) of m00 m01 m10 m11
+};
+
+struct PermDev {
+    int32_t n, c, v, nfix;       // state bits, contiguous low bits, tile bits, fixed bits (n - v)
+    int32_t fixnew[40];          // new bit position of fixed bit i (ascending)
+    int32_t fixold[40];          // old bit position of the same bit
+    uint16_t rdlo[64];           // LDS position (new-local index) of read-order index q, low 6 bits
+    uint16_t rdhi[64];           // ... high part (q >> 6)
+    uint64_t oldhi[64];          // old-index offset of read-order index q >> c
+    uint64_t newhi[64];          // new-index offset of new-local index l >> c
+};
+
+// launchers (svx_kernels.hip)
+hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
+                           const OpDev<float> *od, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
+                            const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t set_kernel_attributes();
+size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits);
+hipError_t launch_permute(int dtype, void *dst, const void *src, const PermDev *pd, uint32_t grid,
+                          size_t lds, hipStream_t s);
+hipError_t launch_swap_words(void *a, void *b, int64_t offa, int64_t offb, int64_t nwords, hipStream_t s);
+hipError_t launch_zero(int dtype, void *st, int64_t n_amps, hipStream_t s);
+hipError_t launch_product(int dtype, void *st, int32_t n, const double *tlo, const double *thi,
+                          int32_t nlo, hipStream_t s);
+hipError_t launch_convert(void *dst, int dt_dst, const void *src, int dt_src, int64_t n, hipStream_t s);
+// reductions: per-workgroup FP64 partials into `part` (device), folded by a second kernel
+struct MargCfg {
+    int32_t n, c, low8, nlow;    // state bits, chunk bits, thread-index bits used, marginal qubits
+    int32_t nsa, nra, nsb, nrb, nshi, pad_;
+    int32_t sa[8], sa_x[8];      // marginal bits inside the thread index (< low8) and their bin bits
+    int32_t ra[8];               // the other thread-index bits
+    int32_t sb[8], sb_x[8];      // marginal bits in [8, c) and their bin bits
+    int32_t rb[8];               // non-marginal bits in [8, c)
+    int32_t shi[8], shi_x[8];    // marginal bits >= c, as chunk-index bits, and their bin bits
+};
+hipError_t launch_marginals(int dtype, const void *st, const MargCfg *cfg_dev, int nbin, double *part,
+                            uint32_t grid, double *out, hipStream_t s);
+hipError_t launch_fid_pass1(int dta, const void *a, int dtb, const void *b, int64_t n_amps, double *part,
+                            uint32_t grid, double *out4, hipStream_t s);
+hipError_t launch_fid_pass2(int dta, const void *a, int dtb, const void *b, int64_t n_amps, const double *coef,
+                            double *part, uint32_t grid, double *out2, hipStream_t s);
+
+}  // namespace svx

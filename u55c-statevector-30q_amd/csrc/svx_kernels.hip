@@ -1,0 +1,706 @@
+// Device kernels of libsvx (gfx950).
+//
+// Pass kernel (k_pass<T, K>): one workgroup owns a 2^K-amplitude tile (one or more QSV2 tiles) in
+// LDS.  It gathers the tile from HBM with 16-byte loads (2^m sub-blocks of 2^(k-m) contiguous
+// amplitudes, the QSV2 tile of kernels/sv/sv_engine.cpp:86-118 / host/sv/model.py:25-40), applies
+// the pass's MUX2 ops group by group -- each group is a run of consecutive ops whose targets lie in
+// 5 tile bits; every thread pulls the 2^5 amplitudes of one slot into registers, applies all of the
+// group's ops there (targets are compile-time register indices, controls are a uniform register
+// mask plus a per-thread predicate), and writes them back -- and stores the tile once.
+// MUX2 semantics (host/sv/isa.py:17-21): pair (i0, i0 | 2^t), bit t of i0 clear, uses M1 iff
+// cmask != 0 and (i0 & cmask) == cmask, else M0.
+#include "svx_internal.h"
+
+namespace svx {
+
+template <typename T>
+struct C2 {
+    T x, y;
+};
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+__device__ __forceinline__ uint32_t lpad(uint32_t e) { return e + (e >> 5); }
+
+__device__ __forceinline__ uint64_t ins0_64(uint64_t x, int p) {
+    const uint64_t lo = x & ((uint64_t(1) << p) - 1);
+    return ((x >> p) << (p + 1)) | lo;
+}
+__device__ __forceinline__ uint32_t ins0_32(uint32_t x, int p) {
+    const uint32_t lo = x & ((1u << p) - 1u);
+    return ((x >> p) << (p + 1)) | lo;
+}
+
+// [a, b] <- M [a, b], M = (m00 m01 m10 m11) as (re, im)
+template <typename T>
+__device__ __forceinline__ void mat2(const T *m, C2<T> &a, C2<T> &b) {
+    const C2<T> x0 = a, x1 = b;
+    a.x = fmaT(m[0], x0.x, fmaT(-m[1], x0.y, fmaT(m[2], x1.x, -m[3] * x1.y)));
+    a.y = fmaT(m[0], x0.y, fmaT(m[1], x0.x, fmaT(m[2], x1.y, m[3] * x1.x)));
+    b.x = fmaT(m[4], x0.x, fmaT(-m[5], x0.y, fmaT(m[6], x1.x, -m[7] * x1.y)));
+    b.y = fmaT(m[4], x0.y, fmaT(m[5], x0.x, fmaT(m[6], x1.y, m[7] * x1.x)));
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(T er, T ei, C2<T> v) {
+    C2<T> r;
+    r.x = fmaT(er, v.x, -ei * v.y);
+    r.y = fmaT(er, v.y, ei * v.x);
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------ ops
+template <typename T, int TB>
+__device__ __forceinline__ void apply_tb(C2<T> (&a)[1 << kR], const OpDev<T> &op, bool ts) {
+    constexpr int NP = 1 << (kR - 1);
+    constexpr int TBIT = 1 << TB;
+    switch (op.kind) {
+    case K_FULL: {
+        T mm[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) mm[e] = op.m0[e];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            mat2(mm, a[r0], a[r0 | TBIT]);
+        }
+        break;
+    }
+    case K_DIAG: {
+        const T d0r = op.m0[0], d0i = op.m0[1], d1r = op.m0[6], d1i = op.m0[7];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            a[r0] = cmul(d0r, d0i, a[r0]);
+            a[r0 | TBIT] = cmul(d1r, d1i, a[r0 | TBIT]);
+        }
+        break;
+    }
+    case K_XSWAP: {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            const C2<T> t0 = a[r0];
+            a[r0] = a[r0 | TBIT];
+            a[r0 | TBIT] = t0;
+        }
+        break;
+    }
+    case K_CFULL: {
+        T me[8];
+        me[0] = ts ? op.m1[0] : T(1);
+        me[1] = ts ? op.m1[1] : T(0);
+        me[2] = ts ? op.m1[2] : T(0);
+        me[3] = ts ? op.m1[3] : T(0);
+        me[4] = ts ? op.m1[4] : T(0);
+        me[5] = ts ? op.m1[5] : T(0);
+        me[6] = ts ? op.m1[6] : T(1);
+        me[7] = ts ? op.m1[7] : T(0);
+        const uint32_t cmr = op.cmr;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            if ((uint32_t(r0) & cmr) == cmr) mat2(me, a[r0], a[r0 | TBIT]);
+        }
+        break;
+    }
+    case K_CDIAG: {
+        const T d0r = ts ? op.m1[0] : T(1), d0i = ts ? op.m1[1] : T(0);
+        const T d1r = ts ? op.m1[6] : T(1), d1i = ts ? op.m1[7] : T(0);
+        const uint32_t cmr = op.cmr;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            if ((uint32_t(r0) & cmr) == cmr) {
+                a[r0] = cmul(d0r, d0i, a[r0]);
+                a[r0 | TBIT] = cmul(d1r, d1i, a[r0 | TBIT]);
+            }
+        }
+        break;
+    }
+    case K_CPHASE: {
+        const T er = ts ? op.m1[6] : T(1), ei = ts ? op.m1[7] : T(0);
+        const uint32_t cmr = op.cmr;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            if ((uint32_t(r0) & cmr) == cmr) a[r0 | TBIT] = cmul(er, ei, a[r0 | TBIT]);
+        }
+        break;
+    }
+    case K_CXSWAP: {
+        const uint32_t cmr = op.cmr;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            if ((uint32_t(r0) & cmr) == cmr) {
+                const C2<T> t0 = a[r0], t1 = a[r0 | TBIT];
+                a[r0].x = ts ? t1.x : t0.x;
+                a[r0].y = ts ? t1.y : t0.y;
+                a[r0 | TBIT].x = ts ? t0.x : t1.x;
+                a[r0 | TBIT].y = ts ? t0.y : t1.y;
+            }
+        }
+        break;
+    }
+    default: {  // K_MUX
+        T m0[8], ms[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            m0[e] = op.m0[e];
+            ms[e] = ts ? op.m1[e] : op.m0[e];
+        }
+        const uint32_t cmr = op.cmr;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+            if ((uint32_t(r0) & cmr) == cmr)
+                mat2(ms, a[r0], a[r0 | TBIT]);
+            else
+                mat2(m0, a[r0], a[r0 | TBIT]);
+        }
+        break;
+    }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_op(C2<T> (&a)[1 << kR], const OpDev<T> &op, bool ts) {
+    static_assert(kR == 5, "register-bit dispatch below assumes 5 register bits");
+    switch (op.tr) {
+    case 0: apply_tb<T, 0>(a, op, ts); break;
+    case 1: apply_tb<T, 1>(a, op, ts); break;
+    case 2: apply_tb<T, 2>(a, op, ts); break;
+    case 3: apply_tb<T, 3>(a, op, ts); break;
+    default: apply_tb<T, 4>(a, op, ts); break;
+    }
+}
+
+// ------------------------------------------------------------------------------------------ pass
+// HBM <-> LDS for one workgroup tile.  Element e of the workgroup tile: QSV2 tile u = e >> k,
+// tile index j = e & (2^k - 1); its state index is tb[u] | sbo[j >> (k-m)] | (j & low mask).
+template <typename T, int K, bool LOAD>
+__device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, const uint64_t *sbo,
+                                          const uint64_t *tb, int k, int lowb, uint32_t tid) {
+    constexpr uint32_t NE = 1u << K;
+    const uint32_t lowmask = (1u << lowb) - 1u, tmask = (1u << k) - 1u;
+    if (sizeof(T) == 4 && lowb >= 1) {
+        // two adjacent amplitudes (16 bytes) per lane and access
+        constexpr int IT = NE / (2 * kNT);
+        constexpr int BATCH = IT < 8 ? IT : 8;
+#pragma unroll 1
+        for (int b0 = 0; b0 < IT; b0 += BATCH) {
+            float4 v[BATCH];
+            uint64_t g[BATCH];
+#pragma unroll
+            for (int i = 0; i < BATCH; ++i) {
+                const uint32_t e = 2u * ((b0 + i) * kNT + tid);
+                const uint32_t j = e & tmask, u = e >> k;
+                g[i] = tb[u] | sbo[j >> lowb] | (j & lowmask);
+            }
+            if (LOAD) {
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i)
+                    v[i] = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(st) + 2 * g[i]);
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) {
+                    const uint32_t e = 2u * ((b0 + i) * kNT + tid), pe = lpad(e);
+                    tile[pe] = C2<T>{T(v[i].x), T(v[i].y)};
+                    tile[pe + 1] = C2<T>{T(v[i].z), T(v[i].w)};
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) {
+                    const uint32_t e = 2u * ((b0 + i) * kNT + tid), pe = lpad(e);
+                    const C2<T> p0 = tile[pe], p1 = tile[pe + 1];
+                    v[i] = make_float4(float(p0.x), float(p0.y), float(p1.x), float(p1.y));
+                }
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i)
+                    *reinterpret_cast<float4 *>(reinterpret_cast<float *>(st) + 2 * g[i]) = v[i];
+            }
+        }
+    } else {
+        constexpr int IT = NE / kNT;
+        constexpr int BATCH = IT < 8 ? IT : 8;
+#pragma unroll 1
+        for (int b0 = 0; b0 < IT; b0 += BATCH) {
+            C2<T> v[BATCH];
+            uint64_t g[BATCH];
+#pragma unroll
+            for (int i = 0; i < BATCH; ++i) {
+                const uint32_t e = (b0 + i) * kNT + tid;
+                const uint32_t j = e & tmask, u = e >> k;
+                g[i] = tb[u] | sbo[j >> lowb] | (j & lowmask);
+            }
+            if (LOAD) {
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) v[i] = st[g[i]];
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) tile[lpad((b0 + i) * kNT + tid)] = v[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) v[i] = tile[lpad((b0 + i) * kNT + tid)];
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) st[g[i]] = v[i];
+            }
+        }
+    }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kNT) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
+                                              const GroupDev *__restrict__ gd,
+                                              const OpDev<T> *__restrict__ od) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr uint32_t NE = 1u << K;
+    constexpr uint32_t NEP = NE + (NE >> 5);
+    C2<T> *tile = reinterpret_cast<C2<T> *>(smem);
+    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + NEP * sizeof(C2<T>));
+    const int k = pd->k, m = pd->m;
+    const int sub = K - k;
+    uint64_t *tb = sbo + (1u << m);
+    const uint32_t tid = threadIdx.x;
+
+    for (uint32_t s = tid; s < (1u << m); s += kNT) {
+        uint64_t o = 0;
+        for (int i = 0; i < m; ++i)
+            if ((s >> i) & 1u) o |= uint64_t(1) << pd->h[i];
+        sbo[s] = o;
+    }
+    for (uint32_t u = tid; u < (1u << sub); u += kNT) {
+        uint64_t x = ((uint64_t(blockIdx.x) << sub) | u) << (k - m);
+        for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
+        tb[u] = x;
+    }
+    __syncthreads();
+    const int lowb = k - m;
+    move_tile<T, K, true>(st, tile, sbo, tb, k, lowb, tid);
+    __syncthreads();
+
+    constexpr int NR = 1 << kR;
+    constexpr uint32_t NSLOT = NE >> kR;
+    const int ng = pd->ngroups, g0 = pd->group0;
+    for (int g = 0; g < ng; ++g) {
+        const GroupDev &G = gd[g0 + g];
+        uint32_t poff[NR];   // padded LDS offset of register r (disjoint bits: pad(a|b) = pad(a)+pad(b))
+        poff[0] = 0;
+#pragma unroll
+        for (int q = 0; q < kR; ++q) {
+            const uint32_t bq = lpad(1u << G.b[q]);
+#pragma unroll
+            for (int r = (1 << q); r < (2 << q); ++r) poff[r] = poff[r - (1 << q)] + bq;
+        }
+        const int nops = G.nops, o0 = G.op0;
+#pragma unroll 1
+        for (uint32_t s = tid; s < NSLOT; s += kNT) {
+            uint32_t base = s;
+#pragma unroll
+            for (int q = 0; q < kR; ++q) base = ins0_32(base, G.bs[q]);
+            const uint32_t pb = lpad(base);
+            C2<T> a[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) a[r] = tile[pb + poff[r]];
+#pragma unroll 1
+            for (int o = 0; o < nops; ++o) {
+                const OpDev<T> &op = od[o0 + o];
+                const bool ts = (base & op.cmt) == op.cmt;
+                apply_op<T>(a, op, ts);
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) tile[pb + poff[r]] = a[r];
+        }
+        __syncthreads();
+    }
+    move_tile<T, K, false>(st, tile, sbo, tb, k, lowb, tid);
+}
+
+// One workgroup runs a whole small state (<= 2^kSmallMaxBits amplitudes), op by op, directly
+// from the MUX2 definition.  Also the independent cross-check of k_pass in the tests.
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_pass_small(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
+                                                    const OpDev<T> *__restrict__ od) {
+    __shared__ C2<T> tile[1u << kSmallMaxBits];
+    const int k = pd->k, m = pd->m, nb = pd->nbits;
+    const int lowb = k - m;
+    const uint32_t NE = 1u << nb, tmask = (1u << k) - 1u, lowmask = (1u << lowb) - 1u;
+    const uint32_t tid = threadIdx.x;
+    auto gidx = [&](uint32_t e) -> uint64_t {
+        const uint32_t j = e & tmask, u = e >> k;
+        uint64_t x = uint64_t(u) << lowb;
+        for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
+        uint64_t o = j & lowmask;
+        for (int i = 0; i < m; ++i)
+            if ((j >> (lowb + i)) & 1u) o |= uint64_t(1) << pd->h[i];
+        return x | o;
+    };
+    for (uint32_t e = tid; e < NE; e += kNT) tile[e] = st[gidx(e)];
+    __syncthreads();
+    const int nops = pd->nops, o0 = pd->op0;
+    for (int o = 0; o < nops; ++o) {
+        const OpDev<T> &op = od[o0 + o];
+        const int t = op.t;
+        const uint32_t cm = uint32_t(op.cmask);
+        for (uint32_t p = tid; p < NE / 2; p += kNT) {
+            const uint32_t i0 = ins0_32(p, t), i1 = i0 | (1u << t);
+            const bool sel = cm != 0 && ((i0 & tmask & cm) == cm);
+            T mm[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) mm[e] = sel ? op.m1[e] : op.m0[e];
+            C2<T> x0 = tile[i0], x1 = tile[i1];
+            mat2(mm, x0, x1);
+            tile[i0] = x0;
+            tile[i1] = x1;
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = tid; e < NE; e += kNT) st[gidx(e)] = tile[e];
+}
+
+size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits) {
+    if (small) return 0;
+    const size_t nep = (size_t(1) << K) + ((size_t(1) << K) >> 5);
+    return nep * elem_bytes + ((size_t(1) << m) + (size_t(1) << (K - k))) * sizeof(uint64_t);
+}
+
+template <typename T>
+static hipError_t launch_pass_t(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
+                                const OpDev<T> *od, uint32_t grid, size_t lds, hipStream_t s) {
+    C2<T> *p = reinterpret_cast<C2<T> *>(st);
+    if (small) {
+        hipLaunchKernelGGL(k_pass_small<T>, dim3(1), dim3(kNT), 0, s, p, pd, od);
+        return hipGetLastError();
+    }
+    switch (K) {
+    case 10: hipLaunchKernelGGL((k_pass<T, 10>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
+    case 11: hipLaunchKernelGGL((k_pass<T, 11>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
+    case 12: hipLaunchKernelGGL((k_pass<T, 12>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
+    case 13: hipLaunchKernelGGL((k_pass<T, 13>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
+    case 14:
+        if constexpr (sizeof(T) == 4) {
+            hipLaunchKernelGGL((k_pass<T, 14>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od);
+            break;
+        } else {
+            return hipErrorInvalidValue;
+        }
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
+                           const OpDev<float> *od, uint32_t grid, size_t lds, hipStream_t s) {
+    return launch_pass_t<float>(K, small, st, pd, gd, od, grid, lds, s);
+}
+hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
+                            const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s) {
+    return launch_pass_t<double>(K, small, st, pd, gd, od, grid, lds, s);
+}
+
+// ------------------------------------------------------------------------------------------ permute
+// Workgroup tile: the new-index bits V (v of them: new bits 0..c-1, the new positions of old bits
+// 0..c-1, then more new bits up to v); the other n - v bits are fixed per workgroup.  Reads walk
+// the tile in old-index order (2^c contiguous old amplitudes per run), writes in new-index order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute(C2<T> *__restrict__ dst, const C2<T> *__restrict__ src,
+                                                 const PermDev *__restrict__ pd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    C2<T> *tile = reinterpret_cast<C2<T> *>(smem);
+    const int c = pd->c, v = pd->v, nfix = pd->nfix;
+    uint64_t nb = 0, ob = 0;
+    const uint64_t wg = blockIdx.x;
+    for (int i = 0; i < nfix; ++i) {
+        if ((wg >> i) & 1u) {
+            nb |= uint64_t(1) << pd->fixnew[i];
+            ob |= uint64_t(1) << pd->fixold[i];
+        }
+    }
+    const uint32_t NE = 1u << v, cm = (1u << c) - 1u;
+    for (uint32_t q = threadIdx.x; q < NE; q += 256) {
+        const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
+        tile[lpad(pos)] = src[ob | (q & cm) | pd->oldhi[q >> c]];
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < NE; l += 256) dst[nb | (l & cm) | pd->newhi[l >> c]] = tile[lpad(l)];
+}
+
+hipError_t launch_permute(int dtype, void *dst, const void *src, const PermDev *pd, uint32_t grid,
+                          size_t lds, hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_permute<float>, dim3(grid), dim3(256), lds, s, (C2<float> *)dst,
+                           (const C2<float> *)src, pd);
+    else
+        hipLaunchKernelGGL(k_permute<double>, dim3(grid), dim3(256), lds, s, (C2<double> *)dst,
+                           (const C2<double> *)src, pd);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------ misc
+__global__ void k_swap_words(uint4 *__restrict__ a, uint4 *__restrict__ b, int64_t n16) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += int64_t(gridDim.x) * blockDim.x) {
+        const uint4 x = a[i], y = b[i];
+        a[i] = y;
+        b[i] = x;
+    }
+}
+
+static uint32_t grid_for(int64_t n, int per_thread) {
+    int64_t g = (n + int64_t(256) * per_thread - 1) / (int64_t(256) * per_thread);
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return uint32_t(g);
+}
+
+hipError_t launch_swap_words(void *a, void *b, int64_t offa, int64_t offb, int64_t nwords, hipStream_t s) {
+    uint4 *pa = reinterpret_cast<uint4 *>(a) + offa * 4, *pb = reinterpret_cast<uint4 *>(b) + offb * 4;
+    const int64_t n16 = nwords * 4;
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_swap_words, dim3(grid_for(n16, 4)), dim3(256), 0, s, pa, pb, n16);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_zero(C2<T> *__restrict__ st, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        st[i] = C2<T>{i == 0 ? T(1) : T(0), T(0)};
+}
+
+hipError_t launch_zero(int dtype, void *st, int64_t n, hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_zero<float>, dim3(grid_for(n, 8)), dim3(256), 0, s, (C2<float> *)st, n);
+    else
+        hipLaunchKernelGGL(k_zero<double>, dim3(grid_for(n, 8)), dim3(256), 0, s, (C2<double> *)st, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_product(C2<T> *__restrict__ st, int64_t n, const double *__restrict__ tlo,
+                          const double *__restrict__ thi, int nlo) {
+    const int64_t lmask = (int64_t(1) << nlo) - 1;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double ar = tlo[2 * (i & lmask)], ai = tlo[2 * (i & lmask) + 1];
+        const double br = thi[2 * (i >> nlo)], bi = thi[2 * (i >> nlo) + 1];
+        st[i] = C2<T>{T(ar * br - ai * bi), T(ar * bi + ai * br)};
+    }
+}
+
+hipError_t launch_product(int dtype, void *st, int32_t n, const double *tlo, const double *thi, int32_t nlo,
+                          hipStream_t s) {
+    const int64_t N = int64_t(1) << n;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_product<float>, dim3(grid_for(N, 8)), dim3(256), 0, s, (C2<float> *)st, N, tlo, thi, nlo);
+    else
+        hipLaunchKernelGGL(k_product<double>, dim3(grid_for(N, 8)), dim3(256), 0, s, (C2<double> *)st, N, tlo, thi, nlo);
+    return hipGetLastError();
+}
+
+template <typename TD, typename TS>
+__global__ void k_convert(C2<TD> *__restrict__ d, const C2<TS> *__restrict__ s, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        d[i] = C2<TD>{TD(s[i].x), TD(s[i].y)};
+}
+
+hipError_t launch_convert(void *dst, int dt_dst, const void *src, int dt_src, int64_t n, hipStream_t s) {
+    const uint32_t g = grid_for(n, 8);
+    if (dt_dst == 0 && dt_src == 0)
+        hipLaunchKernelGGL((k_convert<float, float>), dim3(g), dim3(256), 0, s, (C2<float> *)dst, (const C2<float> *)src, n);
+    else if (dt_dst == 0)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(256), 0, s, (C2<float> *)dst, (const C2<double> *)src, n);
+    else if (dt_src == 0)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, s, (C2<double> *)dst, (const C2<float> *)src, n);
+    else
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(256), 0, s, (C2<double> *)dst, (const C2<double> *)src, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------ FP64 reductions
+// Marginals: workgroup w walks chunks w, w + G, ... of 2^c amplitudes in order.  Inside a chunk the
+// 256 threads own the low 8 index bits; per value of the marginal bits in [8, c) (beta) each thread
+// sums its amplitudes over the other bits in [8, c) (gamma) in order, then a fixed-order tree over
+// the thread bits that are not marginal bits folds the threads of each bin.  All orders are fixed,
+// so the result is bit-reproducible for a given grid.
+template <typename T>
+__global__ __launch_bounds__(256) void k_marg(const C2<T> *__restrict__ st, const MargCfg *__restrict__ cfg,
+                                              double *__restrict__ part) {
+    __shared__ double red[256];
+    __shared__ double acc[256];
+    const int c = cfg->c, low8 = cfg->low8, nlow = cfg->nlow;
+    const int nsa = cfg->nsa, nra = cfg->nra, nsb = cfg->nsb, nrb = cfg->nrb, nshi = cfg->nshi;
+    const uint32_t t = threadIdx.x;
+    const uint32_t nbin = 1u << nlow;
+    for (uint32_t x = t; x < 256; x += 256) acc[x] = 0.0;
+    const bool active = t < (1u << low8);
+    uint32_t ra_lo = 0;   // thread has no ra bits set below the current tree level
+    uint32_t xa = 0;
+    for (int i = 0; i < nsa; ++i)
+        if ((t >> cfg->sa[i]) & 1u) xa |= 1u << cfg->sa_x[i];
+    const uint64_t nchunk = uint64_t(1) << (cfg->n - c);
+    __syncthreads();
+    for (uint64_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        uint32_t xh = 0;
+        for (int i = 0; i < nshi; ++i)
+            if ((ch >> cfg->shi[i]) & 1u) xh |= 1u << cfg->shi_x[i];
+        const uint64_t cbase = ch << c;
+        for (uint32_t beta = 0; beta < (1u << nsb); ++beta) {
+            uint32_t wb = 0, xb = 0;
+            for (int i = 0; i < nsb; ++i)
+                if ((beta >> i) & 1u) {
+                    wb |= 1u << cfg->sb[i];
+                    xb |= 1u << cfg->sb_x[i];
+                }
+            double s = 0.0;
+            if (active) {
+                for (uint32_t gam = 0; gam < (1u << nrb); ++gam) {
+                    uint32_t w = t | wb;
+                    for (int i = 0; i < nrb; ++i)
+                        if ((gam >> i) & 1u) w |= 1u << cfg->rb[i];
+                    const C2<T> a = st[cbase | w];
+                    s = fma(double(a.x), double(a.x), fma(double(a.y), double(a.y), s));
+                }
+            }
+            red[t] = s;
+            __syncthreads();
+            ra_lo = 0;
+            for (int i = 0; i < nra; ++i) {
+                const uint32_t bit = 1u << cfg->ra[i];
+                if (active && !(t & bit) && !(t & ra_lo)) red[t] += red[t | bit];
+                ra_lo |= bit;
+                __syncthreads();
+            }
+            // threads with no ra bit set now hold their bin's sum
+            if (active && !(t & ra_lo)) acc[xa | xb | xh] += red[t];
+            __syncthreads();
+        }
+    }
+    for (uint32_t x = t; x < nbin; x += 256) part[uint64_t(blockIdx.x) * nbin + x] = acc[x];
+}
+
+__global__ __launch_bounds__(256) void k_fold(const double *__restrict__ part, int64_t G, int width,
+                                              double *__restrict__ out, int op_max_mask) {
+    // out[x] = sum_g part[g * width + x] (or max over g where bit x of op_max_mask is set), g in order
+    for (int x = threadIdx.x; x < width; x += 256) {
+        const bool is_max = (op_max_mask >> x) & 1;
+        double s = is_max ? 0.0 : 0.0;
+        for (int64_t g = 0; g < G; ++g) {
+            const double v = part[g * width + x];
+            s = is_max ? fmax(s, v) : s + v;
+        }
+        out[x] = s;
+    }
+}
+
+hipError_t launch_marginals(int dtype, const void *st, const MargCfg *cfg_dev, int nbin, double *part,
+                            uint32_t grid, double *out, hipStream_t s) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_marg<float>, dim3(grid), dim3(256), 0, s, (const C2<float> *)st, cfg_dev, part);
+    else
+        hipLaunchKernelGGL(k_marg<double>, dim3(grid), dim3(256), 0, s, (const C2<double> *)st, cfg_dev, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(256), 0, s, part, int64_t(grid), nbin, out, 0);
+    return hipGetLastError();
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_fid1(const C2<TA> *__restrict__ a, const C2<TB> *__restrict__ b, int64_t n,
+                                              double *__restrict__ part) {
+    __shared__ double red[4][256];
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        s0 = fma(ar, br, fma(ai, bi, s0));       // Re <a|b>
+        s1 = fma(ar, bi, fma(-ai, br, s1));      // Im <a|b>
+        s2 = fma(ar, ar, fma(ai, ai, s2));
+        s3 = fma(br, br, fma(bi, bi, s3));
+    }
+    const int t = threadIdx.x;
+    red[0][t] = s0; red[1][t] = s1; red[2][t] = s2; red[3][t] = s3;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + w];
+        __syncthreads();
+    }
+    if (t < 4) part[uint64_t(blockIdx.x) * 4 + t] = red[t][0];
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_fid2(const C2<TA> *__restrict__ a, const C2<TB> *__restrict__ b, int64_t n,
+                                              const double *__restrict__ coef, double *__restrict__ part) {
+    __shared__ double rs[256], rm[256];
+    // coef: ph.re, ph.im, 1/|a|, 1/|b|;  e = b/|b| - ph a/|a|
+    const double pr = coef[0], pi = coef[1], ia = coef[2], ib = coef[3];
+    double d2 = 0, mx = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double ar = a[i].x * ia, ai = a[i].y * ia;
+        const double er = b[i].x * ib - (pr * ar - pi * ai);
+        const double ei = b[i].y * ib - (pr * ai + pi * ar);
+        const double m2 = er * er + ei * ei;
+        d2 += m2;
+        mx = fmax(mx, m2);
+    }
+    const int t = threadIdx.x;
+    rs[t] = d2;
+    rm[t] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            rs[t] += rs[t + w];
+            rm[t] = fmax(rm[t], rm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[uint64_t(blockIdx.x) * 2] = rs[0];
+        part[uint64_t(blockIdx.x) * 2 + 1] = rm[0];
+    }
+}
+
+hipError_t launch_fid_pass1(int dta, const void *a, int dtb, const void *b, int64_t n, double *part, uint32_t grid,
+                            double *out4, hipStream_t s) {
+    if (dta == 0 && dtb == 0)
+        hipLaunchKernelGGL((k_fid1<float, float>), dim3(grid), dim3(256), 0, s, (const C2<float> *)a, (const C2<float> *)b, n, part);
+    else if (dta == 0)
+        hipLaunchKernelGGL((k_fid1<float, double>), dim3(grid), dim3(256), 0, s, (const C2<float> *)a, (const C2<double> *)b, n, part);
+    else if (dtb == 0)
+        hipLaunchKernelGGL((k_fid1<double, float>), dim3(grid), dim3(256), 0, s, (const C2<double> *)a, (const C2<float> *)b, n, part);
+    else
+        hipLaunchKernelGGL((k_fid1<double, double>), dim3(grid), dim3(256), 0, s, (const C2<double> *)a, (const C2<double> *)b, n, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(256), 0, s, part, int64_t(grid), 4, out4, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_fid_pass2(int dta, const void *a, int dtb, const void *b, int64_t n, const double *coef,
+                            double *part, uint32_t grid, double *out2, hipStream_t s) {
+    if (dta == 0 && dtb == 0)
+        hipLaunchKernelGGL((k_fid2<float, float>), dim3(grid), dim3(256), 0, s, (const C2<float> *)a, (const C2<float> *)b, n, coef, part);
+    else if (dta == 0)
+        hipLaunchKernelGGL((k_fid2<float, double>), dim3(grid), dim3(256), 0, s, (const C2<float> *)a, (const C2<double> *)b, n, coef, part);
+    else if (dtb == 0)
+        hipLaunchKernelGGL((k_fid2<double, float>), dim3(grid), dim3(256), 0, s, (const C2<double> *)a, (const C2<float> *)b, n, coef, part);
+    else
+        hipLaunchKernelGGL((k_fid2<double, double>), dim3(grid), dim3(256), 0, s, (const C2<double> *)a, (const C2<double> *)b, n, coef, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(256), 0, s, part, int64_t(grid), 2, out2, 2);
+    return hipGetLastError();
+}
+
+hipError_t set_kernel_attributes() {
+    const int big = 160 * 1024;
+    hipError_t e = hipSuccess;
+#define SVX_ATTR(fn)                                                                              \
+    if (e == hipSuccess) e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),               \
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    SVX_ATTR((k_pass<float, 10>)) SVX_ATTR((k_pass<float, 11>)) SVX_ATTR((k_pass<float, 12>))
+    SVX_ATTR((k_pass<float, 13>)) SVX_ATTR((k_pass<float, 14>))
+    SVX_ATTR((k_pass<double, 10>)) SVX_ATTR((k_pass<double, 11>)) SVX_ATTR((k_pass<double, 12>))
+    SVX_ATTR((k_pass<double, 13>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
+#undef SVX_ATTR
+    return e;
+}
+
+}  // namespace svx

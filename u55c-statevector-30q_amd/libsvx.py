@@ -1,0 +1,232 @@
+"""ctypes binding of libsvx.so (include/svx.h) -- the Python side of the drop-in boundary.
+
+Device buffers are torch tensors (complex64 / complex128 on a ROCm device); only their
+data_ptr() and the current HIP stream cross the boundary.  Negative return codes become the
+exceptions the reference raises at the same conditions (AssertionError for a bad magic,
+host/sv/model.py:9; ValueError for bad arguments; RuntimeError for capacity / runtime failures,
+host/sv/runtime.py:220-221).  The library must be built (python csrc/build.py); there is no
+fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libsvx.so")
+
+C64, C128 = 0, 1
+SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -1, -2, -3, -4, -5, -6
+
+SYMBOLS = ["svx_init", "svx_run_segment", "svx_engine", "svx_prepare", "svx_run_prepared",
+           "svx_prepared_info", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
+           "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync", "svx_last_error",
+           "svx_version"]
+
+
+class SvxCaps(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("cu_count", ctypes.c_int32),
+                ("lds_per_block", ctypes.c_int64), ("hbm_bytes", ctypes.c_int64),
+                ("wavefront", ctypes.c_int32), ("max_k_c64", ctypes.c_int32),
+                ("max_k_c128", ctypes.c_int32), ("max_high", ctypes.c_int32),
+                ("arch", ctypes.c_char * 64)]
+
+
+_lib = None
+
+_VP, _I32, _I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PD = ctypes.POINTER(ctypes.c_double)
+_SIGS = {
+    "svx_init": ([_I32, ctypes.POINTER(SvxCaps)], _I32),
+    "svx_run_segment": ([_VP, _I64, _I32, _I32, _PI32, _I64, _VP], _I32),
+    "svx_engine": ([_VP, _I64, _VP], _I32),
+    "svx_prepare": ([_PI32, _I64, _I32, _I32, ctypes.POINTER(_I64)], _I32),
+    "svx_run_prepared": ([_VP, _I64, _VP], _I32),
+    "svx_prepared_info": ([_I64, _PI32, _PI32], _I32),
+    "svx_permute": ([_VP, _VP, _I32, _PI32, _I32, _VP], _I32),
+    "svx_swap_words": ([_VP, _VP, _I64, _I64, _I64, _VP], _I32),
+    "svx_zero_state": ([_VP, _I32, _I32, _VP], _I32),
+    "svx_product_state": ([_VP, _I32, _I32, _PD, _VP], _I32),
+    "svx_reduce_f64": ([_VP, _I32, _I32, _PI32, _I32, _PD, _PD, _VP], _I32),
+    "svx_fidelity_f64": ([_VP, _I32, _VP, _I32, _I32, _PD, _VP], _I32),
+    "svx_convert": ([_VP, _I32, _VP, _I32, _I64, _VP], _I32),
+    "svx_sync": ([_VP], _I32),
+    "svx_last_error": ([], ctypes.c_char_p),
+    "svx_version": ([], ctypes.c_char_p),
+}
+
+
+def load(path=LIB_PATH):
+    """Load libsvx.so (torch is imported first so one HIP runtime serves both)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python u55c-statevector-30q_amd/csrc/build.py` "
+                           "(or __graft_entry__.build()); there is no non-HIP fallback")
+    import torch  # noqa: F401  (binds libamdhip64 before libsvx resolves it)
+    lib = ctypes.CDLL(path)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc == 0:
+        return
+    msg = _lib.svx_last_error().decode(errors="replace")
+    if rc == SVX_E_MAGIC:
+        raise AssertionError(msg)
+    if rc in (SVX_E_ARG, SVX_E_FORMAT):
+        raise ValueError(msg)
+    raise RuntimeError(f"libsvx error {rc}: {msg}")
+
+
+def _i32(a):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+    return a, a.ctypes.data_as(_PI32)
+
+
+def dtype_code(t):
+    import torch
+    if t.dtype == torch.complex64:
+        return C64
+    if t.dtype == torch.complex128:
+        return C128
+    raise ValueError(f"state must be complex64 or complex128, got {t.dtype}")
+
+
+def _stream(stream):
+    if stream is not None:
+        return ctypes.c_void_p(int(stream))
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _nbits(t):
+    n = t.numel().bit_length() - 1
+    if t.numel() != 1 << n:
+        raise ValueError("state length must be a power of two")
+    return n
+
+
+def _dev(t):
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("state must be a contiguous device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def init(device=0):
+    lib = load()
+    caps = SvxCaps()
+    _check(lib.svx_init(device, ctypes.byref(caps)))
+    return caps
+
+
+def version():
+    return load().svx_version().decode()
+
+
+def run_segment(state, words, n_slices=1, stream=None):
+    """svx_run_segment: every 2^L slice of `state` (n_slices of them) runs the QSV2 segment."""
+    lib = load()
+    w, wp = _i32(words)
+    n_slice = state.numel() // n_slices
+    _check(lib.svx_run_segment(_dev(state), n_slice, n_slices, dtype_code(state), wp, w.size, _stream(stream)))
+
+
+def engine(buf_u8, ins_off, stream=None):
+    """svx_engine: the reference kernel ABI -- a uint8 device buffer holding the 2^L complex64
+    slice followed by the instruction region at 256-bit word ins_off."""
+    lib = load()
+    _check(lib.svx_engine(_dev(buf_u8), int(ins_off), _stream(stream)))
+
+
+def prepare(words, dtype, state_bits):
+    lib = load()
+    w, wp = _i32(words)
+    h = ctypes.c_int64(0)
+    _check(lib.svx_prepare(wp, w.size, dtype, state_bits, ctypes.byref(h)))
+    return h.value
+
+
+def run_prepared(state, handle, stream=None):
+    lib = load()
+    _check(lib.svx_run_prepared(_dev(state), handle, _stream(stream)))
+
+
+def prepared_info(handle):
+    lib = load()
+    a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+    _check(lib.svx_prepared_info(handle, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def permute(dst, src, perm, stream=None):
+    """svx_permute: dst = src with perm[new_bit] = old_bit (bit-exact, out of place)."""
+    lib = load()
+    if dst.dtype != src.dtype or dst.numel() != src.numel():
+        raise ValueError("dst and src must have the same dtype and size")
+    n = _nbits(src)
+    p, pp = _i32(perm)
+    if p.size != n:
+        raise ValueError("perm must have one entry per qubit")
+    _check(lib.svx_permute(_dev(dst), _dev(src), n, pp, dtype_code(src), _stream(stream)))
+
+
+def swap_words(a, b, offa, offb, nwords, stream=None):
+    lib = load()
+    _check(lib.svx_swap_words(_dev(a), _dev(b), offa, offb, nwords, _stream(stream)))
+
+
+def zero_state(state, stream=None):
+    lib = load()
+    _check(lib.svx_zero_state(_dev(state), _nbits(state), dtype_code(state), _stream(stream)))
+
+
+def product_state(state, amps, stream=None):
+    """amps: (n, 2) complex -- qubit q = amps[q, 0] |0> + amps[q, 1] |1>."""
+    lib = load()
+    n = _nbits(state)
+    a = np.ascontiguousarray(np.asarray(amps, dtype=np.complex128).reshape(n, 2).view(np.float64).reshape(n, 4))
+    _check(lib.svx_product_state(_dev(state), n, dtype_code(state), a.ctypes.data_as(_PD), _stream(stream)))
+
+
+def reduce_f64(state, fmap=(), stream=None):
+    """FP64 norm and the marginal distribution of logical qubits 0..len(fmap)-1 (at physical fmap[i])."""
+    lib = load()
+    n = _nbits(state)
+    f, fp = _i32(list(fmap) if len(fmap) else [0])
+    nl = len(fmap)
+    norm = ctypes.c_double(0)
+    marg = np.zeros(1 << nl, dtype=np.float64)
+    _check(lib.svx_reduce_f64(_dev(state), n, dtype_code(state), fp, nl, ctypes.byref(norm),
+                              marg.ctypes.data_as(_PD), _stream(stream)))
+    return norm.value, marg
+
+
+def fidelity_f64(ref, psi, stream=None):
+    """(F, 1-F without cancellation, max|da|, l2) of psi against the reference ref."""
+    lib = load()
+    if ref.numel() != psi.numel():
+        raise ValueError("states differ in size")
+    out = np.zeros(4, dtype=np.float64)
+    _check(lib.svx_fidelity_f64(_dev(ref), dtype_code(ref), _dev(psi), dtype_code(psi), _nbits(ref),
+                                out.ctypes.data_as(_PD), _stream(stream)))
+    return dict(fidelity=float(out[0]), one_minus_f=float(out[1]), max_abs_err=float(out[2]), l2=float(out[3]))
+
+
+def convert(dst, src, stream=None):
+    lib = load()
+    if dst.numel() != src.numel():
+        raise ValueError("size mismatch")
+    _check(lib.svx_convert(_dev(dst), dtype_code(dst), _dev(src), dtype_code(src), src.numel(), _stream(stream)))
+
+
+def sync(stream=None):
+    lib = load()
+    _check(lib.svx_sync(_stream(stream)))

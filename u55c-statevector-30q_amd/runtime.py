@@ -1,0 +1,178 @@
+"""SVRuntime: the reference runtime's interface (host/sv/runtime.py:83-334) on one MI355X.
+
+`run(qc, ...)` compiles, zeroes the state on the device, launches every segment through libsvx
+(the drop-in for the XRT run call), executes remaps as one permutation sweep each (instead of the
+reference's transposition loop over sv_swap / host copies, host/sv/runtime.py:235-284), reads the
+state back, samples shots and returns the reference's dict shape:
+dict(timing, P, L, k, n_run, stats, counts[, state]).
+
+target="mi355x" (default) compiles with the MI355X retarget (one slice, 2^13-amplitude LDS tile, up
+to 7 gathered high qubits, DAG-aware pass packing); target="reference" keeps the reference's
+plan (P slices of 2^L, k <= 12, m <= max_high) and runs its stream unchanged -- P slices are one
+device buffer whose top log2(P) index bits are the reference's bank qubits.
+"""
+import time
+
+import numpy as np
+
+from . import compiler, libsvx
+from .compiler import compile_circuit, compile_gpu
+
+KMAX = 12          # host/sv/runtime.py:26-32
+LPREF = 17
+LMIN = 12
+AMP = 8
+
+
+def sample_probabilities(state, shots, fmap, n, seed=None, chunk_bits=12):
+    """Born sampling (host/sv/runtime.py:47-80): per-chunk sums (|a|^2 in the state's precision,
+    accumulated in FP64 across chunks), choose chunks, then positions; physical indices map to
+    logical bitstrings with fmap[l] = physical position of logical qubit l."""
+    if shots <= 0:
+        return {}
+    rng = np.random.default_rng(seed)
+    N = state.size
+    cb = min(chunk_bits, N.bit_length() - 1)
+    f = state.view(np.float32 if state.dtype == np.complex64 else np.float64).reshape(-1, 2 << cb)
+    sums = np.einsum("ij,ij->i", f, f).astype(np.float64)
+    cs = np.cumsum(sums)
+    u = rng.random(shots) * cs[-1]
+    ch = np.searchsorted(cs, u, side="right").clip(0, sums.size - 1)
+    rem = u - np.where(ch > 0, cs[ch - 1], 0.0)
+    phys = np.empty(shots, dtype=np.int64)
+    for c in np.unique(ch):
+        sel = np.nonzero(ch == c)[0]
+        off = int(c) << cb
+        p = np.cumsum(np.abs(state[off:off + (1 << cb)].astype(np.complex128)) ** 2)
+        phys[sel] = off + np.searchsorted(p, rem[sel], side="right").clip(0, (1 << cb) - 1)
+    logical = np.zeros(shots, dtype=np.int64)
+    for l in range(n):
+        logical |= ((phys >> fmap[l]) & 1) << l
+    vals, cnt = np.unique(logical, return_counts=True)
+    return {int(v): int(c) for v, c in zip(vals, cnt)}
+
+
+class SVRuntime:
+    def __init__(self, device=0, n_engines=16, max_high=None, target="mi355x", dtype="c64"):
+        import torch
+        self.torch = torch
+        self.device = device
+        self.caps = libsvx.init(device)
+        self.n_engines = n_engines
+        self.target = target
+        self.max_high = max_high
+        self.dtype = torch.complex64 if dtype == "c64" else torch.complex128
+        self.st = None
+        self.tmp = None
+        self.log = []
+
+    # ------------------------------------------------------------------ planning
+    def plan(self, n, p_max=None):
+        """Reference slice planning (host/sv/runtime.py:158-171) for target="reference"."""
+        if self.target != "reference":
+            return 1, n, n
+        cap = p_max or self.n_engines
+        P = 1
+        while P * 2 <= min(cap, self.n_engines) and n - (P * 2).bit_length() + 1 >= LPREF:
+            P *= 2
+
+        def fits(P):
+            L = max(n, LMIN + P.bit_length() - 1) - (P.bit_length() - 1)
+            return (1 << L) * AMP + (4 << 20) <= (512 << 20)
+        while not fits(P) and P * 2 <= cap and n - (P * 2).bit_length() + 1 >= LMIN:
+            P *= 2
+        n_run = max(n, LMIN + P.bit_length() - 1)
+        return P, n_run, n_run - (P.bit_length() - 1)
+
+    def admit(self, n, dtype=None):
+        """Footprint admission: the state (2^n amplitudes) plus one permutation buffer must fit."""
+        dt = self.dtype if dtype is None else dtype
+        es = 8 if dt == self.torch.complex64 else 16
+        need = (2 << n) * es
+        free, total = self.torch.cuda.mem_get_info(self.device)
+        return need <= free + (self.st.numel() * self.st.element_size() * 2 if self.st is not None else 0)
+
+    def _alloc(self, n):
+        torch = self.torch
+        if self.st is None or self.st.numel() != (1 << n) or self.st.dtype != self.dtype:
+            self.st = None
+            self.tmp = None
+            self.st = torch.empty(1 << n, dtype=self.dtype, device=f"cuda:{self.device}")
+
+    def _tmp(self):
+        if self.tmp is None or self.tmp.numel() != self.st.numel() or self.tmp.dtype != self.st.dtype:
+            self.tmp = self.torch.empty_like(self.st)
+        return self.tmp
+
+    def compile(self, qc, k=None, fuse=True, tiling=True, initial_map=None, p_max=None):
+        n = qc.num_qubits
+        if self.target == "reference":
+            P, n_run, L = self.plan(n, p_max)
+            kk = min(k or KMAX, L)
+            qrun = qc.padded(n_run) if n_run != n else qc
+            mh = 4 if self.max_high is None else self.max_high
+            prog, fmap, stats = compile_circuit(qrun, L, kk, fuse_1q=fuse, tiling=tiling,
+                                                initial_map=initial_map, max_high=mh)
+            return prog, fmap, stats, P, n_run, L, kk
+        kk = k or compiler.GPU_K
+        prog, fmap, stats = compile_gpu(qc, k=kk, max_high=self.max_high, fuse_1q=fuse)
+        return prog, fmap, stats, 1, n, n, min(kk, n)
+
+    def execute(self, prog, n_run, P, stream=None, timing=None):
+        """Run a compiled program on the resident state (already initialised)."""
+        torch = self.torch
+        started = False
+        T = timing if timing is not None else {}
+        for kind, payload in prog:
+            if kind == "remap" and not started:
+                continue          # |0...0> is invariant under qubit permutations (host/sv/runtime.py:308-309)
+            a = time.perf_counter()
+            if kind == "segment":
+                started = True
+                libsvx.run_segment(self.st, payload, n_slices=P, stream=stream)
+                torch.cuda.synchronize(self.device)
+                T["T_kernel"] = T.get("T_kernel", 0.0) + time.perf_counter() - a
+            else:
+                tmp = self._tmp()
+                libsvx.permute(tmp, self.st, payload, stream=stream)
+                self.st, self.tmp = tmp, self.st
+                torch.cuda.synchronize(self.device)
+                T["T_exchange"] = T.get("T_exchange", 0.0) + time.perf_counter() - a
+        return T
+
+    def logical_state(self, fmap, n_run):
+        """State in logical order (host/sv/runtime.py:327-330), permuted on the device."""
+        if list(fmap) == list(range(n_run)):
+            return self.st
+        tmp = self._tmp()
+        libsvx.permute(tmp, self.st, [fmap[l] for l in range(n_run)])
+        return tmp
+
+    def run(self, qc, shots=1024, k=None, p_max=None, fuse=True, tiling=True, initial_map=None,
+            return_state=True, seed=None):
+        torch = self.torch
+        n = qc.num_qubits
+        t0 = time.perf_counter()
+        prog, fmap, stats, P, n_run, L, kk = self.compile(qc, k, fuse, tiling, initial_map, p_max)
+        t1 = time.perf_counter()
+        self._alloc(n_run)
+        t1b = time.perf_counter()
+        libsvx.zero_state(self.st)
+        torch.cuda.synchronize(self.device)
+        t2 = time.perf_counter()
+        T = dict(T_compile=t1 - t0, T_h2d=t2 - t1b, T_alloc=t1b - t1, T_kernel=0.0, T_exchange=0.0)
+        self.execute(prog, n_run, P, timing=T)
+        a = time.perf_counter()
+        host = self.st.cpu().numpy()
+        b = time.perf_counter()
+        T["T_d2h"] = b - a
+        counts = sample_probabilities(host, shots, fmap, n, seed)
+        c = time.perf_counter()
+        T["T_post"] = c - b
+        T["T_e2e"] = T["T_compile"] + T["T_h2d"] + T["T_kernel"] + T["T_exchange"] + T["T_d2h"] + T["T_post"]
+        out = dict(timing=T, P=P, L=L, k=kk, n_run=n_run, stats=stats.__dict__, counts=counts)
+        if return_state:
+            from_dev = self.logical_state(fmap, n_run).cpu().numpy()
+            out["state"] = from_dev[: 1 << n]
+            T["T_state_reorder"] = time.perf_counter() - c
+        return out
