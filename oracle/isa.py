@@ -17,3 +17,5 @@ MAX_HIGH = 4
 # This project's extension for the MI355X retarget: a pass with m > 4 high qubits is followed by
 # an 8-word record [OP_HIGHX, h4, h5, h6, h7, h8, h9, h10]; reference streams (m <= 4) never hold it.
 OP_HIGHX = 0x48
+# Fused form of this project: [OP_DENSE, f, q0..q4, 0] + 2 * 4^f float32 words (dense U, row-major re, im).
+OP_DENSE = 2

@@ -19,7 +19,7 @@ reference module itself is pinned by the committed vectors in tests/golden/ when
 """
 import numpy as np
 
-from .isa import MAGIC, HDR_WORDS, PASS_WORDS, OP_WORDS, OP_MUX2, OP_HIGHX
+from .isa import MAGIC, HDR_WORDS, PASS_WORDS, OP_WORDS, OP_MUX2, OP_HIGHX, OP_DENSE
 
 
 def decode(seg, ctype=np.complex64):
@@ -43,6 +43,17 @@ def decode(seg, ctype=np.complex64):
             pos += PASS_WORDS
         ops = []
         for _ in range(nops):
+            if int(w[pos]) == OP_DENSE:
+                # this project's fused form: [OP_DENSE, f, q0..q4, 0] + 2 * 4^f f32 words of U
+                f = int(w[pos + 1])
+                qs = [int(v) for v in w[pos + 2:pos + 2 + f]]
+                fl = np.ascontiguousarray(w[pos + 8:pos + 8 + (2 << (2 * f))]).view(np.float32)
+                U = np.empty(1 << (2 * f), dtype=ctype)
+                U.real = fl[0::2]
+                U.imag = fl[1::2]
+                ops.append((OP_DENSE, qs, 0, U.reshape(1 << f, 1 << f), None))
+                pos += 8 + (2 << (2 * f))
+                continue
             rec = w[pos:pos + OP_WORDS]
             pos += OP_WORDS
             fl = np.ascontiguousarray(rec[8:24]).view(np.float32)
@@ -103,6 +114,23 @@ def apply_op_tile(T, code, t, cmask, M0, M1):
         V[:, :, 1, :] = np.where(where[None], y1, V[:, :, 1, :])
 
 
+def apply_dense_tile(T, qs, U):
+    """This project's fused record: a dense 2^f x 2^f U on tile qubits qs (index sum_i bit(qs[i]) << i)
+    applied to every tile (rows of T), in place, in T's dtype.  Pinned against the product of the
+    MUX2 ops it replaces (tests/test_oracle.py)."""
+    nt, size = T.shape
+    k = size.bit_length() - 1
+    f = len(qs)
+    V = T.reshape((nt,) + (2,) * k)
+    axes = [1 + (k - 1 - qs[f - 1 - i]) for i in range(f)]
+    Ut = U.reshape((2,) * (2 * f))
+    out = np.tensordot(V, Ut, axes=(axes, list(range(f, 2 * f))))
+    # out axes: rest of V (in order), then U's output bits (f-1 .. 0)
+    rest = [a for a in range(k + 1) if a not in axes]
+    order = rest + axes
+    T[:] = np.transpose(out, np.argsort(order)).reshape(nt, size)
+
+
 def run_segment(state, P, seg, ctype=None):
     """host/sv/model.py:59-71 -- run one segment on each of P contiguous 2^L slices, in place."""
     ctype = state.dtype if ctype is None else ctype
@@ -114,7 +142,10 @@ def run_segment(state, P, seg, ctype=None):
             sl = state[p << L:(p + 1) << L]
             T = sl[idx]
             for code, t, cm, M0, M1 in ops:
-                apply_op_tile(T, code, t, cm, M0, M1)
+                if code == OP_DENSE:
+                    apply_dense_tile(T, t, M0)
+                else:
+                    apply_op_tile(T, code, t, cm, M0, M1)
             sl[idx] = T
     return state
 

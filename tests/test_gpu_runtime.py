@@ -33,13 +33,19 @@ def test_families_vs_dense(gpu, target):
     assert worst < 1e-5
 
 
-def test_c128_runtime_matches_dense_tightly(gpu):
+def test_c128_runtime_matches_golden_model_tightly(gpu):
+    """complex128 runs of the same records match the complex128 golden model to FMA rounding; the
+    records carry float32 matrices (host/sv/isa.py:21), so against the exact circuit the bound is the
+    float32 payload rounding (~1e-7)."""
     from u55c_statevector_30q_amd.runtime import SVRuntime
     rt = SVRuntime(dtype="c128")
     for qc in (C.make("qft", 10), C.u3_cx_layers(11, 8, 3), C.cp_ladder(9)):
-        ref = dense.simulate(qc.tuples(), qc.num_qubits)
+        n = qc.num_qubits
+        prog, fmap, st = K.compile_gpu(qc)
+        gold = model.run_program(prog, n, 1, fmap, ctype=np.complex128)
         out = rt.run(qc, shots=0)
-        assert judge.metrics(ref, out["state"])["max_abs_err"] < 1e-12
+        assert judge.metrics(gold, out["state"])["max_abs_err"] < 1e-12
+        assert judge.metrics(dense.simulate(qc.tuples(), n), out["state"])["max_abs_err"] < 1e-6
 
 
 def test_sampling_tvd(gpu):

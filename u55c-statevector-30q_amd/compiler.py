@@ -358,7 +358,10 @@ def _is_identity(Mx):
 
 
 def op_cost(op):
-    """Rough VALU cost of one MUX2 per amplitude (a general 2x2 = 8 FMA per amplitude)."""
+    """Rough VALU cost of one MUX2 per amplitude (a general 2x2 = 8 FMA per amplitude); a fused
+    block carries its own cost."""
+    if hasattr(op, "cost"):
+        return op.cost
     M1 = np.asarray(op.M1)
     if op.ctrls and _is_identity(op.M0):
         if np.count_nonzero(M1 - np.diag(np.diag(M1))) == 0:
@@ -505,9 +508,176 @@ def split_passes(words):
     pos = 8
     for _ in range(int(w[2])):
         m, nops = int(w[pos]), int(w[pos + 3])
-        length = 8 + (8 if m > 4 else 0) + 24 * nops
+        length = 8 + (8 if m > 4 else 0)
+        for _ in range(nops):
+            code = int(w[pos + length])
+            length += 24 if code == OP_MUX2 else 8 + (2 << (2 * int(w[pos + length + 1])))
         seg = np.concatenate([w[:8], w[pos:pos + length]]).astype(np.int32)
         seg[2] = 1
         out.append(seg)
         pos += length
     return out
+
+
+# ------------------------------------------------------------------ fused dense-unitary form
+# Record of this project's fused form (the reference stream has OP_MUX2 only, host/sv/isa.py:30):
+#   [OP_DENSE, f, q0, q1, q2, q3, q4, 0] followed by 2 * 4^f float32 words: the 2^f x 2^f complex64
+#   matrix U row-major as (re, im), basis index sum_i bit(q_i) << i over the listed tile qubits.
+OP_DENSE = 2
+DENSE_MAX_F = 5
+
+
+def dense_words(Um, tq):
+    f = len(tq)
+    U = np.asarray(Um, dtype=np.complex64).reshape(1 << f, 1 << f)
+    q = list(tq) + [0] * (5 - f)
+    fl = np.empty(2 << (2 * f), dtype=np.float32)
+    fl[0::2] = U.real.ravel()
+    fl[1::2] = U.imag.ravel()
+    return [OP_DENSE, f] + q + [0] + [int(x) for x in fl.view(np.int32)]
+
+
+def op_dense(op, qubits):
+    """Dense matrix of one MUX2 op on the ordered qubit list (index sum_i bit(qubits[i]) << i)."""
+    f = len(qubits)
+    pos = {q: i for i, q in enumerate(qubits)}
+    bt = 1 << pos[op.t]
+    cm = 0
+    for c in op.ctrls:
+        cm |= 1 << pos[c]
+    O = np.zeros((1 << f, 1 << f), dtype=np.complex128)
+    for i0 in range(1 << f):
+        if i0 & bt:
+            continue
+        M = op.M1 if (cm and (i0 & cm) == cm) else op.M0
+        i1 = i0 | bt
+        O[i0, i0], O[i0, i1] = M[0, 0], M[0, 1]
+        O[i1, i0], O[i1, i1] = M[1, 0], M[1, 1]
+    return O
+
+
+class Block:
+    """A fused cluster: ordered qubits, its ops, and whether it is emitted as one dense op."""
+
+    def __init__(self, qubits, ops):
+        self.qubits = list(qubits)
+        self.ops = ops
+        self.dense = False
+
+    @property
+    def q(self):
+        return tuple(self.qubits)
+
+    def matrix(self):
+        U = np.eye(1 << len(self.qubits), dtype=np.complex128)
+        for op in self.ops:
+            U = op_dense(op, self.qubits) @ U
+        return U
+
+
+def fuse_ops(ops, width):
+    """Greedy DAG clustering: a block grows by the ready op that adds the fewest new qubits
+    (earliest first on ties) while its qubit set stays <= width."""
+    nq = len(ops)
+    last = {}
+    preds = [0] * nq
+    succs = [[] for _ in range(nq)]
+    for i, op in enumerate(ops):
+        deps = set()
+        for q in op.q:
+            if q in last:
+                deps.add(last[q])
+            last[q] = i
+        preds[i] = len(deps)
+        for d in deps:
+            succs[d].append(i)
+    ready = {i for i in range(nq) if preds[i] == 0}
+    blocks = []
+    while ready:
+        S = set()
+        members = []
+        while True:
+            best, bcost = None, None
+            for i in ready:
+                add = len(set(ops[i].q) - S)
+                if len(S) + add > width:
+                    continue
+                key = (add, i)
+                if bcost is None or key < bcost:
+                    best, bcost = i, key
+            if best is None:
+                break
+            ready.discard(best)
+            members.append(best)
+            S |= set(ops[best].q)
+            for s in succs[best]:
+                preds[s] -= 1
+                if preds[s] == 0:
+                    ready.add(s)
+        blocks.append(Block(sorted(S), [ops[i] for i in members]))
+    return blocks
+
+
+def dense_cost(f):
+    """Matrix-core cost of one dense f-qubit op in VALU-FMA units per amplitude: 8 * 2^f real flops
+    per amplitude at twice the non-packed FP32 VALU rate."""
+    return 8.0 * (1 << f) / 2.0 / 2.0
+
+
+def compile_fused(qc, k=GPU_K, width=4, max_high=None, fuse_1q=True, max_ops=GPU_MAX_OPS, max_cost=None):
+    """Fused dense-unitary form: ops are clustered into blocks of <= width qubits; a block whose
+    MUX2 cost exceeds the dense cost of its padded width becomes one OP_DENSE record (padded to
+    `width` qubits with idle tile qubits), the rest stay MUX2.  Blocks are packed into passes by the
+    same window rule as MUX2 ops.  Returns (prog, final_map, stats) with stats.n_dense added."""
+    n = qc.num_qubits
+    k = min(k, n)
+    if max_high is None:
+        max_high = max(0, min(GPU_MAX_HIGH, k - 6)) if k >= 8 else max(0, k - 1)
+    stats = Stats()
+    ops, relab = circuit_to_ops(qc, stats, fuse_1q, max_width=k)
+    blocks = fuse_ops(ops, width)
+    units = []
+    for b in blocks:
+        mux_cost = sum(op_cost(op) for op in b.ops)
+        if len(b.qubits) >= 2 and mux_cost > dense_cost(width):
+            b.dense = True
+            b.cost = dense_cost(width)
+            units.append(b)
+        else:
+            for op in b.ops:
+                units.append(op)
+    # pass packing over units (blocks and plain ops) with the window scheduler
+    passes = schedule_window(units, n, k, stats, max_high, max_ops, max_cost)
+    words = [MAGIC, n, len(passes), k, 0, 0, 0, 0]
+    n_dense = 0
+    for p in passes:
+        phys_set = {q for _, ph in p for q in ph}
+        m, hi = pass_window(phys_set, k, max_high)
+        tmap = {q: q for q in phys_set if q < k - m}
+        for i, q in enumerate(hi):
+            tmap[q] = k - m + i
+        h = list(hi) + [0] * (11 - len(hi))
+        words += [m, h[0], h[1], len(p), h[2], h[3], 0, 0]
+        if m > 4:
+            words += [OP_HIGHX] + h[4:11]
+        used_tile = set(tmap.values())
+        for u, ph in p:
+            if isinstance(u, Block):
+                tq = [tmap[q] for q in u.qubits]
+                U = u.matrix()
+                pad = [t for t in range(k) if t not in tq and t not in used_tile][:max(0, width - len(tq))]
+                if len(pad) < width - len(tq):
+                    pad += [t for t in range(k) if t not in tq and t not in pad][:width - len(tq) - len(pad)]
+                for _ in pad:
+                    U = np.kron(np.eye(2), U)
+                words += dense_words(U, tq + pad)
+                n_dense += 1
+            else:
+                words += _op_words(u, [tmap[q] for q in ph])
+    arr = np.asarray(words, dtype=np.int64)
+    seg = (arr & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+    stats.n_segments = 1
+    stats.n_passes = len(passes)
+    stats.n_dense = n_dense
+    final_map = [relab[l] for l in range(n)]
+    return [("segment", seg)], final_map, stats
