@@ -129,7 +129,12 @@ def main():
         e1.record(stream)
     torch.cuda.synchronize()
     per_pass_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
-    mean_ms = float(np.mean(per_pass_ms))
+    kernels = [libsvx.prepared_kernel(h, 0) for h in single]
+    by_kernel = {}
+    for kname, t in zip(kernels, per_pass_ms):
+        by_kernel.setdefault(kname, []).append(t)
+    dominant = max(by_kernel, key=lambda kk: sum(by_kernel[kk]))
+    mean_ms = float(np.mean(by_kernel[dominant]))
     achieved = bytes_per_pass / (mean_ms * 1e-3) / 1e9
     per_pass_gbs = [bytes_per_pass / (t * 1e-3) / 1e9 for t in per_pass_ms]
 
@@ -169,8 +174,10 @@ def main():
         "frac_of_peak": round(value / PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_pass<float,13>", "bytes_per_launch": bytes_per_pass,
-                     "mean_launch_ms": round(mean_ms, 4)},
+                     "kernel": dominant, "bytes_per_launch": bytes_per_pass,
+                     "mean_launch_ms": round(mean_ms, 4), "launches": len(by_kernel[dominant]),
+                     "per_kernel": {kk: {"launches": len(v), "mean_ms": round(float(np.mean(v)), 4),
+                                         "total_ms": round(float(np.sum(v)), 3)} for kk, v in by_kernel.items()}},
         "per_pass_gbs": {"min": round(min(per_pass_gbs), 1), "median": round(float(np.median(per_pass_gbs)), 1),
                          "max": round(max(per_pass_gbs), 1),
                          "n_at_or_above_70pct": int(sum(g >= 0.7 * PEAK_GBS for g in per_pass_gbs)),

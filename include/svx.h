@@ -72,6 +72,10 @@ int svx_run_prepared(void *state, int64_t handle, void *stream);
 /* Number of passes (kernel launches) of a prepared segment, and the kernel index of pass p. */
 int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches);
 
+/* Name of the kernel that runs pass `pass` of a prepared segment ("k_pass<float,13>", "k_pass_dma<13,4>",
+ * ..., "none" for a pass without effective ops), for profiling reports.  No reference counterpart. */
+int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
+
 /* Qubit permutation perm[new_bit] = old_bit of a 2^n state, out of place (dst != src), bit-exact.
  * Replaces the reference's remap paths: model.remap (host/sv/model.py:74-78), the host transpose
  * of local qubits (host/sv/runtime.py:261-268) and the sv_swap bank exchange

@@ -38,6 +38,11 @@ def build(verbose=True):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    # every kernel's host stub must be defined (an undefined one only shows up at dlopen)
+    nm = subprocess.run(["nm", "-C", "--undefined-only", tmp], capture_output=True, text=True)
+    missing = [l for l in nm.stdout.splitlines() if "svx::" in l]
+    if missing:
+        raise RuntimeError("undefined libsvx symbols:\n" + "\n".join(missing))
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)

@@ -13,9 +13,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -189,7 +191,11 @@ bool same(const T *a, const T *b) {
 
 struct LaunchInfo {
     bool small;
+    bool dma;             // double-buffered complex64 kernel (k_pass_dma)
+    bool pers;            // persistent single-buffer complex64 kernel (k_pass_pers)
+    int dma_variant;      // 0: default worker layout; 100: 5-bit register groups, 4 worker waves
     int K;
+    uint64_t ntiles;
     uint32_t grid;
     size_t lds;
     int64_t pass_index;   // index into the PassDev array
@@ -199,6 +205,7 @@ struct LaunchInfo {
 struct Prepared {
     int dtype = 0, nbits = 0;
     std::vector<int32_t> words;
+    std::string pick;           // SVX_PASS_KERNEL at preparation time
     void *dev = nullptr;        // [PassDev x P][GroupDev x G][OpDev<T> x O]
     size_t pass_off = 0, group_off = 0, op_off = 0;
     std::vector<LaunchInfo> launches;
@@ -228,30 +235,36 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         return fail(SVX_E_CAPACITY, "tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS tile limit 2^" +
                                         std::to_string(maxK));
     const bool small = nbits <= kSmallMaxBits;
-    int K = 0;
+    // kernel choice per pass.  The double-buffered DMA kernel (k_pass_dma) needs complex64, one QSV2
+    // tile per LDS buffer (K = k in {12, 13}) and enough tiles to keep every CU busy; it overlaps
+    // HBM traffic with the gate math but its 4-bit register groups cost more per op, so "auto" uses
+    // it for passes whose estimated VALU cost is small and the single-buffer kernel (k_pass, 5-bit
+    // groups) otherwise.  SVX_PASS_KERNEL=v1|dma|dma5|pers|auto forces a choice.
+    const char *env = std::getenv("SVX_PASS_KERNEL");
+    const std::string pick = env ? env : "auto";
+    const bool dma_ok = dtype == SVX_C64 && (k == 12 || k == 13) && nbits >= k + 9;
+    if ((pick == "dma" || pick == "pers" || pick == "dma5") && !dma_ok && !small)
+        return fail(SVX_E_ARG, "SVX_PASS_KERNEL=" + pick + " needs complex64, k in {12, 13} and >= 2^(k+9) amplitudes");
+    int Kv1 = 0;
     if (!small) {
-        K = std::max(k, kMinK);
-        if (K - k > kMaxTileSub) K = k + kMaxTileSub;
-        if (K < kMinK) K = kMinK;   // unreachable with kMaxTileSub >= kMinK
-        if (K > nbits) K = nbits;
-        if (K > maxK) K = maxK;
-        if (K < k) return fail(SVX_E_CAPACITY, "tile larger than the state");
+        Kv1 = std::max(k, kMinK);
+        if (Kv1 - k > kMaxTileSub) Kv1 = k + kMaxTileSub;
+        if (Kv1 < kMinK) Kv1 = kMinK;
+        if (Kv1 > nbits) Kv1 = nbits;
+        if (Kv1 > maxK) Kv1 = maxK;
+        if (Kv1 < k) return fail(SVX_E_CAPACITY, "tile larger than the state");
     }
+    DeviceState *ds = nullptr;
+    int rc0 = get_ds(&ds);
+    if (rc0) return rc0;
     std::vector<PassDev> passes;
     std::vector<GroupDev> groups;
     std::vector<OpDev<T>> ops;
+    std::vector<LaunchInfo> launches;
     for (const RawPass &rp : seg.passes) {
-        PassDev pd{};
-        pd.k = k;
-        pd.m = rp.m;
-        pd.K = small ? nbits : K;
-        pd.nbits = nbits;
-        for (int i = 0; i < rp.m; ++i) pd.h[i] = pd.hs[i] = rp.h[i];
-        std::sort(pd.hs, pd.hs + rp.m);
-        pd.group0 = (int)groups.size();
-        pd.op0 = (int)ops.size();
         // classify
         std::vector<OpDev<T>> pops;
+        double cost = 0;
         for (const RawOp &ro : rp.ops) {
             OpDev<T> od{};
             for (int e = 0; e < 8; ++e) {
@@ -277,16 +290,72 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             } else {
                 od.kind = K_MUX;
             }
+            static const double kcost[] = {8, 4, 1, 8, 4, 2, 2, 8};
+            cost += kcost[od.kind];
             pops.push_back(od);
         }
+        int mode;   // 0 small, 1 single-buffer, 2 DMA, 3 persistent
+        if (small) mode = 0;
+        else if (pick == "v1" || !dma_ok) mode = 1;
+        else if (pick == "pers") mode = 3;
+        else if (pick == "dma" || pick == "dma5") mode = 2;
+        else mode = cost <= kDmaMaxCost ? 2 : 1;
+        const bool dma5 = mode == 2 && pick == "dma5" && k == 13;
+        const int K = mode == 0 ? nbits : (mode == 1 ? Kv1 : k);
+        const int R = mode == 2 ? (dma5 ? 5 : 4) : (mode == 3 ? (k == 12 ? 4 : 5) : kR);
+        PassDev pd{};
+        pd.k = k;
+        pd.m = rp.m;
+        pd.K = K;
+        pd.nbits = nbits;
+        for (int i = 0; i < rp.m; ++i) pd.h[i] = pd.hs[i] = rp.h[i];
+        std::sort(pd.hs, pd.hs + rp.m);
+        if (mode >= 2) {
+            // uniform part of the DMA loader's element offsets: tile elements 128 j + (< 128)
+            const int lowb = k - rp.m;
+            for (int j = 0; j < (1 << k) / 128 && j < 64; ++j) {
+                const uint64_t e0 = uint64_t(128) * j;
+                uint64_t u = e0 & ((uint64_t(1) << lowb) - 1);
+                const uint64_t sb = e0 >> lowb;
+                for (int i = 0; i < rp.m; ++i)
+                    if ((sb >> i) & 1) u |= uint64_t(1) << rp.h[i];
+                pd.dma_u[j] = u;
+            }
+        }
+        pd.group0 = (int)groups.size();
+        pd.op0 = (int)ops.size();
         pd.nops = (int)pops.size();
-        if (small) {
+        LaunchInfo li{};
+        li.small = mode == 0;
+        li.dma = mode == 2;
+        li.pers = mode == 3;
+        li.dma_variant = dma5 ? 100 : 0;
+        li.K = K;
+        li.pass_index = (int64_t)passes.size();
+        li.empty = pd.nops == 0;
+        li.ntiles = uint64_t(1) << (nbits - K);
+        if (mode == 0) {
+            li.grid = 1;
+            li.lds = 0;
+        } else if (mode == 3) {
+            li.grid = (uint32_t)std::min<uint64_t>(li.ntiles, 2 * (uint64_t)ds->caps.cu_count);
+            li.lds = pass_pers_lds_bytes(K, rp.m);
+        } else if (mode == 2) {
+            const uint64_t per_cu = K == 12 ? 2 : 1;   // LDS: two 2^K complex64 buffers per workgroup
+            li.grid = (uint32_t)std::min<uint64_t>(li.ntiles, per_cu * (uint64_t)ds->caps.cu_count);
+            li.lds = pass_dma_lds_bytes(K, rp.m);
+        } else {
+            li.grid = (uint32_t)li.ntiles;
+            li.lds = pass_lds_bytes(K, rp.m, k, (int)sizeof(T) * 2, false, nbits);
+        }
+        launches.push_back(li);
+        if (mode == 0) {
             for (auto &o : pops) ops.push_back(o);
             pd.ngroups = 0;
             passes.push_back(pd);
             continue;
         }
-        // groups: maximal runs of ops whose targets fit kR register bits
+        // groups: maximal runs of ops whose targets fit R register bits
         size_t i = 0;
         while (i < pops.size()) {
             std::vector<int> tg;
@@ -294,34 +363,39 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             for (; j < pops.size(); ++j) {
                 const int t = pops[j].t;
                 if (std::find(tg.begin(), tg.end(), t) == tg.end()) {
-                    if ((int)tg.size() == kR) break;
+                    if ((int)tg.size() == R) break;
                     tg.push_back(t);
                 }
             }
             // register bits: the targets, then filler bits from the top of the workgroup tile
             std::vector<int> rb = tg;
-            for (int b = K - 1; b >= 0 && (int)rb.size() < kR; --b)
+            for (int b = K - 1; b >= 0 && (int)rb.size() < R; --b)
                 if (std::find(rb.begin(), rb.end(), b) == rb.end()) rb.push_back(b);
             GroupDev gd{};
             gd.nops = (int)(j - i);
             gd.op0 = (int)ops.size();
             uint32_t regmask = 0;
-            for (int q = 0; q < kR; ++q) {
+            for (int q = 0; q < R; ++q) {
                 gd.b[q] = gd.bs[q] = rb[q];
                 regmask |= 1u << rb[q];
             }
-            std::sort(gd.bs, gd.bs + kR);
+            std::sort(gd.bs, gd.bs + R);
             for (size_t o = i; o < j; ++o) {
                 OpDev<T> od = pops[o];
                 od.tr = (int)(std::find(rb.begin(), rb.end(), od.t) - rb.begin());
                 uint32_t cmr = 0;
-                for (int q = 0; q < kR; ++q)
+                for (int q = 0; q < R; ++q)
                     if ((uint32_t)od.cmask >> rb[q] & 1u) cmr |= 1u << q;
                 od.cmr = cmr;
                 od.cmt = (uint32_t)od.cmask & ~regmask;
                 if (od.cmask == 0 || od.kind <= K_XSWAP) {
                     od.cmr = 0;
                     od.cmt = 0;
+                }
+                // one control, inside the register group: the touched pairs are known at compile time
+                if (od.cmt == 0 && __builtin_popcount(od.cmr) == 1) {
+                    if (od.kind == K_CXSWAP) od.kind = K_CXSWAP1;
+                    else if (od.kind == K_CPHASE) od.kind = K_CPHASE1;
                 }
                 ops.push_back(od);
             }
@@ -344,22 +418,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     HIPCHK(hipMalloc(&pr->dev, total));
     HIPCHK(hipMemcpy(pr->dev, host.data(), total, hipMemcpyHostToDevice));
     g_prep_bytes += total;
-    pr->launches.clear();
-    for (size_t p = 0; p < passes.size(); ++p) {
-        LaunchInfo li{};
-        li.small = small;
-        li.K = passes[p].K;
-        li.pass_index = (int64_t)p;
-        li.empty = passes[p].nops == 0;
-        if (small) {
-            li.grid = 1;
-            li.lds = 0;
-        } else {
-            li.grid = (uint32_t)(uint64_t(1) << (nbits - K));
-            li.lds = pass_lds_bytes(K, passes[p].m, k, (int)sizeof(T) * 2, false, nbits);
-        }
-        pr->launches.push_back(li);
-    }
+    pr->launches = launches;
     return 0;
 }
 
@@ -369,13 +428,15 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     DeviceState *ds = nullptr;
     int rc = get_ds(&ds);
     if (rc) return rc;
+    const char *kenv = std::getenv("SVX_PASS_KERNEL");
+    const std::string pick = kenv ? kenv : "auto";
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
-                                        (uint64_t)ds->caps.device * 97u);
+                                        (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);
     auto range = g_prep_index.equal_range(h);
     for (auto it = range.first; it != range.second; ++it) {
         Prepared *p = g_prepared[it->second - 1].get();
-        if (p && p->dtype == dtype && p->nbits == nbits && p->device == ds->caps.device &&
+        if (p && p->dtype == dtype && p->nbits == nbits && p->device == ds->caps.device && p->pick == pick &&
             (int64_t)p->words.size() == nw && std::memcmp(p->words.data(), w, nw * 4) == 0) {
             *handle = it->second;
             if (out) *out = p;
@@ -391,6 +452,7 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     pr->nbits = nbits;
     pr->device = ds->caps.device;
     pr->words.assign(w, w + nw);
+    pr->pick = pick;
     rc = dtype == SVX_C64 ? build_prepared<float>(seg, nbits, dtype, pr.get())
                           : build_prepared<double>(seg, nbits, dtype, pr.get());
     if (rc) return rc;
@@ -411,7 +473,13 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
     for (const LaunchInfo &li : pr->launches) {
         if (li.empty) continue;
         hipError_t e;
-        if (pr->dtype == SVX_C64)
+        if (li.pers)
+            e = launch_pass_pers(li.K, state, pd + li.pass_index, gd,
+                                 reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
+        else if (li.dma)
+            e = launch_pass_dma(li.K + li.dma_variant, state, pd + li.pass_index, gd,
+                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
+        else if (pr->dtype == SVX_C64)
             e = launch_pass_c64(li.K, li.small, state, pd + li.pass_index, gd,
                                 reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.grid, li.lds, s);
         else
@@ -543,6 +611,24 @@ int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches) {
     for (auto &li : pr->launches) nl += li.empty ? 0 : 1;
     if (n_passes) *n_passes = (int32_t)pr->launches.size();
     if (n_launches) *n_launches = nl;
+    return 0;
+}
+
+int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
+        return fail(SVX_E_ARG, "unknown prepared-segment handle");
+    Prepared *pr = g_prepared[handle - 1].get();
+    if (pass < 0 || pass >= (int32_t)pr->launches.size() || !name || cap < 1) return fail(SVX_E_ARG, "bad pass index");
+    const LaunchInfo &li = pr->launches[pass];
+    const char *T = pr->dtype == SVX_C64 ? "float" : "double";
+    std::string k;
+    if (li.empty) k = "none";
+    else if (li.small) k = std::string("k_pass_small<") + T + ">";
+    else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
+    else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5>" : ",4>");
+    else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + ">";
+    std::snprintf(name, cap, "%s", k.c_str());
     return 0;
 }
 

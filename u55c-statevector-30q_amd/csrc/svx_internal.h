@@ -13,6 +13,7 @@ constexpr int kMaxKc64 = 14;     // 2^14 complex64 = 128 KiB (+ padding) of LDS
 constexpr int kMaxKc128 = 13;    // 2^13 complex128 = 128 KiB (+ padding)
 constexpr int kSmallMaxBits = 9; // states of <= 2^9 amplitudes run in one workgroup (k_pass_small)
 constexpr int kMaxTileSub = 8;   // at most 2^8 QSV2 tiles per workgroup tile
+constexpr double kDmaMaxCost = 40; // passes up to ~5 general 2x2 ops (VALU FMA per amplitude) use k_pass_dma
 
 // op kinds (host classification of the MUX2 record; exact comparisons on the typed values)
 enum OpKind : int32_t {
@@ -24,6 +25,8 @@ enum OpKind : int32_t {
     K_CPHASE = 5,  // controlled, M0 == I, M1 = diag(1, e)
     K_CXSWAP = 6,  // controlled, M0 == I, M1 exact X
     K_MUX = 7,     // controlled, general: selected pairs M1, others M0
+    K_CXSWAP1 = 8, // controlled exact X, one control in the register group, none outside
+    K_CPHASE1 = 9, // controlled diag(1, e), one control in the register group, none outside
 };
 
 struct PassDev {
@@ -31,6 +34,7 @@ struct PassDev {
     int32_t ngroups, group0, nops, op0;
     int32_t h[kMaxM];            // physical bit of tile qubit k-m+i
     int32_t hs[kMaxM];           // h sorted ascending (tile-id deposit)
+    uint64_t dma_u[64];          // DMA kernel: element offset of tile elements 128 j .. from bits >= 7
 };
 
 struct GroupDev {
@@ -65,6 +69,15 @@ hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                             const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t set_kernel_attributes();
+// double-buffered complex64 pass kernel (one QSV2 tile of 2^K per LDS buffer, K in {12, 13});
+// register groups of 4 (K = 12) or 5 (K = 13) tile bits
+size_t pass_dma_lds_bytes(int K, int m);
+// persistent single-buffer complex64 pass kernel (two workgroups per CU)
+size_t pass_pers_lds_bytes(int K, int m);
+hipError_t launch_pass_pers(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+                            uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+                           uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
 size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits);
 hipError_t launch_permute(int dtype, void *dst, const void *src, const PermDev *pd, uint32_t grid,
                           size_t lds, hipStream_t s);

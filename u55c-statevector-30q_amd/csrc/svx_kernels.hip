@@ -51,129 +51,184 @@ __device__ __forceinline__ C2<T> cmul(T er, T ei, C2<T> v) {
 }
 
 // ------------------------------------------------------------------------------------------ ops
-template <typename T, int TB>
-__device__ __forceinline__ void apply_tb(C2<T> (&a)[1 << kR], const OpDev<T> &op, bool ts) {
-    constexpr int NP = 1 << (kR - 1);
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_full(C2<T> (&a)[1 << R], const T *m0) {
     constexpr int TBIT = 1 << TB;
-    switch (op.kind) {
-    case K_FULL: {
-        T mm[8];
+    T mm[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) mm[e] = op.m0[e];
+    for (int e = 0; e < 8; ++e) mm[e] = m0[e];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
-            mat2(mm, a[r0], a[r0 | TBIT]);
-        }
-        break;
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        mat2(mm, a[r0], a[r0 | TBIT]);
     }
-    case K_DIAG: {
-        const T d0r = op.m0[0], d0i = op.m0[1], d1r = op.m0[6], d1i = op.m0[7];
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_diag(C2<T> (&a)[1 << R], const T *m0) {
+    constexpr int TBIT = 1 << TB;
+    const T d0r = m0[0], d0i = m0[1], d1r = m0[6], d1i = m0[7];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        a[r0] = cmul(d0r, d0i, a[r0]);
+        a[r0 | TBIT] = cmul(d1r, d1i, a[r0 | TBIT]);
+    }
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_xswap(C2<T> (&a)[1 << R]) {
+    constexpr int TBIT = 1 << TB;
+#pragma unroll
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        const C2<T> t0 = a[r0];
+        a[r0] = a[r0 | TBIT];
+        a[r0 | TBIT] = t0;
+    }
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_cfull(C2<T> (&a)[1 << R], const T *m1, uint32_t cmr, bool ts) {
+    constexpr int TBIT = 1 << TB;
+    T me[8];
+    me[0] = ts ? m1[0] : T(1);
+    me[1] = ts ? m1[1] : T(0);
+    me[2] = ts ? m1[2] : T(0);
+    me[3] = ts ? m1[3] : T(0);
+    me[4] = ts ? m1[4] : T(0);
+    me[5] = ts ? m1[5] : T(0);
+    me[6] = ts ? m1[6] : T(1);
+    me[7] = ts ? m1[7] : T(0);
+#pragma unroll
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if ((uint32_t(r0) & cmr) == cmr) mat2(me, a[r0], a[r0 | TBIT]);
+    }
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_cdiag(C2<T> (&a)[1 << R], const T *m1, uint32_t cmr, bool ts) {
+    constexpr int TBIT = 1 << TB;
+    const T d0r = ts ? m1[0] : T(1), d0i = ts ? m1[1] : T(0);
+    const T d1r = ts ? m1[6] : T(1), d1i = ts ? m1[7] : T(0);
+#pragma unroll
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if ((uint32_t(r0) & cmr) == cmr) {
             a[r0] = cmul(d0r, d0i, a[r0]);
             a[r0 | TBIT] = cmul(d1r, d1i, a[r0 | TBIT]);
         }
-        break;
     }
-    case K_XSWAP: {
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_cphase(C2<T> (&a)[1 << R], const T *m1, uint32_t cmr, bool ts) {
+    constexpr int TBIT = 1 << TB;
+    const T er = ts ? m1[6] : T(1), ei = ts ? m1[7] : T(0);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if ((uint32_t(r0) & cmr) == cmr) a[r0 | TBIT] = cmul(er, ei, a[r0 | TBIT]);
+    }
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_cxswap(C2<T> (&a)[1 << R], uint32_t cmr, bool ts) {
+    constexpr int TBIT = 1 << TB;
+#pragma unroll
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if ((uint32_t(r0) & cmr) == cmr) {
+            const C2<T> t0 = a[r0], t1 = a[r0 | TBIT];
+            a[r0].x = ts ? t1.x : t0.x;
+            a[r0].y = ts ? t1.y : t0.y;
+            a[r0 | TBIT].x = ts ? t0.x : t1.x;
+            a[r0 | TBIT].y = ts ? t0.y : t1.y;
+        }
+    }
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_mux(C2<T> (&a)[1 << R], const T *m0p, const T *m1, uint32_t cmr, bool ts) {
+    constexpr int TBIT = 1 << TB;
+    T m0[8], ms[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        m0[e] = m0p[e];
+        ms[e] = ts ? m1[e] : m0p[e];
+    }
+#pragma unroll
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if ((uint32_t(r0) & cmr) == cmr)
+            mat2(ms, a[r0], a[r0 | TBIT]);
+        else
+            mat2(m0, a[r0], a[r0 | TBIT]);
+    }
+}
+
+// controlled ops whose single control is register bit CB (no per-thread control): only the
+// pairs with bit CB set are touched, chosen at compile time
+template <typename T, int R, int TB, int CB>
+__device__ __forceinline__ void op_cxswap1(C2<T> (&a)[1 << R]) {
+    constexpr int TBIT = 1 << TB;
+#pragma unroll
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if (r0 & (1 << CB)) {
             const C2<T> t0 = a[r0];
             a[r0] = a[r0 | TBIT];
             a[r0 | TBIT] = t0;
         }
-        break;
     }
-    case K_CFULL: {
-        T me[8];
-        me[0] = ts ? op.m1[0] : T(1);
-        me[1] = ts ? op.m1[1] : T(0);
-        me[2] = ts ? op.m1[2] : T(0);
-        me[3] = ts ? op.m1[3] : T(0);
-        me[4] = ts ? op.m1[4] : T(0);
-        me[5] = ts ? op.m1[5] : T(0);
-        me[6] = ts ? op.m1[6] : T(1);
-        me[7] = ts ? op.m1[7] : T(0);
-        const uint32_t cmr = op.cmr;
+}
+template <typename T, int R, int TB, int CB>
+__device__ __forceinline__ void op_cphase1(C2<T> (&a)[1 << R], const T *m1) {
+    constexpr int TBIT = 1 << TB;
+    const T er = m1[6], ei = m1[7];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
-            if ((uint32_t(r0) & cmr) == cmr) mat2(me, a[r0], a[r0 | TBIT]);
-        }
-        break;
+    for (int p = 0; p < (1 << (R - 1)); ++p) {
+        const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
+        if (r0 & (1 << CB)) a[r0 | TBIT] = cmul(er, ei, a[r0 | TBIT]);
     }
-    case K_CDIAG: {
-        const T d0r = ts ? op.m1[0] : T(1), d0i = ts ? op.m1[1] : T(0);
-        const T d1r = ts ? op.m1[6] : T(1), d1i = ts ? op.m1[7] : T(0);
-        const uint32_t cmr = op.cmr;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
-            if ((uint32_t(r0) & cmr) == cmr) {
-                a[r0] = cmul(d0r, d0i, a[r0]);
-                a[r0 | TBIT] = cmul(d1r, d1i, a[r0 | TBIT]);
-            }
-        }
+}
+template <typename T, int R, int TB>
+__device__ __forceinline__ void op_ctl1(C2<T> (&a)[1 << R], const OpDev<T> &op, bool swap) {
+    // control register bit = index of the single set bit of cmr
+    const int cb = __builtin_ctz(op.cmr | (1u << 31));
+#define SVX_CB(CBV)                                                                        \
+    case CBV:                                                                              \
+        if constexpr (CBV != TB && CBV < R) {                                              \
+            if (swap) op_cxswap1<T, R, TB, CBV>(a); else op_cphase1<T, R, TB, CBV>(a, op.m1); \
+        }                                                                                  \
         break;
-    }
-    case K_CPHASE: {
-        const T er = ts ? op.m1[6] : T(1), ei = ts ? op.m1[7] : T(0);
-        const uint32_t cmr = op.cmr;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
-            if ((uint32_t(r0) & cmr) == cmr) a[r0 | TBIT] = cmul(er, ei, a[r0 | TBIT]);
-        }
-        break;
-    }
-    case K_CXSWAP: {
-        const uint32_t cmr = op.cmr;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
-            if ((uint32_t(r0) & cmr) == cmr) {
-                const C2<T> t0 = a[r0], t1 = a[r0 | TBIT];
-                a[r0].x = ts ? t1.x : t0.x;
-                a[r0].y = ts ? t1.y : t0.y;
-                a[r0 | TBIT].x = ts ? t0.x : t1.x;
-                a[r0 | TBIT].y = ts ? t0.y : t1.y;
-            }
-        }
-        break;
-    }
-    default: {  // K_MUX
-        T m0[8], ms[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            m0[e] = op.m0[e];
-            ms[e] = ts ? op.m1[e] : op.m0[e];
-        }
-        const uint32_t cmr = op.cmr;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int r0 = ((p >> TB) << (TB + 1)) | (p & (TBIT - 1));
-            if ((uint32_t(r0) & cmr) == cmr)
-                mat2(ms, a[r0], a[r0 | TBIT]);
-            else
-                mat2(m0, a[r0], a[r0 | TBIT]);
-        }
-        break;
-    }
+    switch (cb) { SVX_CB(0) SVX_CB(1) SVX_CB(2) SVX_CB(3) SVX_CB(4) default: break; }
+#undef SVX_CB
+}
+
+// one switch over (kind, target register bit): code = kind * 8 + tr, set by the host
+template <typename T, int R, int TB>
+__device__ __forceinline__ void apply_kind(C2<T> (&a)[1 << R], const OpDev<T> &op, bool ts, int kind) {
+    switch (kind) {
+    case K_FULL: op_full<T, R, TB>(a, op.m0); break;
+    case K_DIAG: op_diag<T, R, TB>(a, op.m0); break;
+    case K_XSWAP: op_xswap<T, R, TB>(a); break;
+    case K_CFULL: op_cfull<T, R, TB>(a, op.m1, op.cmr, ts); break;
+    case K_CDIAG: op_cdiag<T, R, TB>(a, op.m1, op.cmr, ts); break;
+    case K_CPHASE: op_cphase<T, R, TB>(a, op.m1, op.cmr, ts); break;
+    case K_CXSWAP: op_cxswap<T, R, TB>(a, op.cmr, ts); break;
+    case K_CXSWAP1: op_ctl1<T, R, TB>(a, op, true); break;
+    case K_CPHASE1: op_ctl1<T, R, TB>(a, op, false); break;
+    default: op_mux<T, R, TB>(a, op.m0, op.m1, op.cmr, ts); break;
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void apply_op(C2<T> (&a)[1 << kR], const OpDev<T> &op, bool ts) {
-    static_assert(kR == 5, "register-bit dispatch below assumes 5 register bits");
+template <typename T, int R>
+__device__ __forceinline__ void apply_op(C2<T> (&a)[1 << R], const OpDev<T> &op, bool ts) {
+    static_assert(R == 4 || R == 5, "register groups hold 4 or 5 tile bits");
+    const int kind = op.kind;
     switch (op.tr) {
-    case 0: apply_tb<T, 0>(a, op, ts); break;
-    case 1: apply_tb<T, 1>(a, op, ts); break;
-    case 2: apply_tb<T, 2>(a, op, ts); break;
-    case 3: apply_tb<T, 3>(a, op, ts); break;
-    default: apply_tb<T, 4>(a, op, ts); break;
+    case 0: apply_kind<T, R, 0>(a, op, ts, kind); break;
+    case 1: apply_kind<T, R, 1>(a, op, ts, kind); break;
+    case 2: apply_kind<T, R, 2>(a, op, ts, kind); break;
+    case 3: apply_kind<T, R, 3>(a, op, ts, kind); break;
+    default:
+        if constexpr (R == 5) apply_kind<T, R, 4>(a, op, ts, kind);
+        break;
     }
 }
 
@@ -306,7 +361,7 @@ __global__ __launch_bounds__(kNT) void k_pass(C2<T> *__restrict__ st, const Pass
             for (int o = 0; o < nops; ++o) {
                 const OpDev<T> &op = od[o0 + o];
                 const bool ts = (base & op.cmt) == op.cmt;
-                apply_op<T>(a, op, ts);
+                apply_op<T, kR>(a, op, ts);
             }
 #pragma unroll
             for (int r = 0; r < NR; ++r) tile[pb + poff[r]] = a[r];
@@ -314,6 +369,256 @@ __global__ __launch_bounds__(kNT) void k_pass(C2<T> *__restrict__ st, const Pass
         __syncthreads();
     }
     move_tile<T, K, false>(st, tile, sbo, tb, k, lowb, tid);
+}
+
+// ------------------------------------------------------------------------------------------ pass, DMA
+// Persistent, double-buffered pass kernel for complex64 with one QSV2 tile per LDS buffer (K = k).
+// Wave 0 is the loader: it streams the next tile straight into the idle LDS buffer with 16-byte
+// global_load_lds while waves 1..4 (the workers) apply the pass's groups to the current buffer and
+// store it back; the loader waits for its own loads only, the workers never wait on their stores,
+// so HBM reads, the gate math and HBM writes of consecutive tiles overlap.  The two roles run
+// separate loops with the same barrier count per tile.
+//
+// LDS layout: 16-byte slots, logical pair P (elements 2P, 2P+1) in slot P ^ ((P >> 4) & 15); the
+// byte address of element e is swz(e), an XOR-linear map, so swz(base | off) = swz(base) ^ swz(off)
+// for the disjoint bit sets of a register group.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ __forceinline__ uint32_t swz(uint32_t e) {
+    return (((e >> 1) ^ ((e >> 5) & 15u)) << 4) | ((e & 1u) << 3);
+}
+
+__device__ __forceinline__ void lds_barrier() {
+    // LDS writes of this wave complete, then the workgroup barrier; global stores stay in flight
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+
+template <int K, int R, int NW>
+__global__ __launch_bounds__(64 + NW) void k_pass_dma(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
+                                                   const GroupDev *__restrict__ gd,
+                                                   const OpDev<float> *__restrict__ od, uint64_t ntiles) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr uint32_t NE = 1u << K, NS = NE / 2, BUF = NE * 8;
+    constexpr uint32_t kNTD = 64 + NW;       // loader wave + NW worker threads
+    static_assert((NE >> R) % NW == 0, "every worker owns the same number of register slots");
+    constexpr int NR = 1 << R;
+    constexpr uint32_t NSLOT = NE >> R;
+    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + 2 * BUF);
+    const int m = pd->m, lowb = K - m;
+    const uint32_t lowmask = (1u << lowb) - 1u;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t q = tid; q < (1u << m); q += kNTD) {
+        uint64_t o = 0;
+        for (int i = 0; i < m; ++i)
+            if ((q >> i) & 1u) o |= uint64_t(1) << pd->h[i];
+        sbo[q] = o;
+    }
+    uint64_t t = blockIdx.x;
+    const uint64_t G = gridDim.x;
+    if (t >= ntiles) return;
+    auto tile_base = [&](uint64_t tt) {
+        uint64_t x = tt << lowb;
+        for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
+        return x;
+    };
+    const int ng = pd->ngroups, g0 = pd->group0;
+    lds_barrier();
+    if (tid < 64) {
+        // ------------------------------------------------ loader wave
+        const uint32_t lane = tid;
+        // DMA instruction j moves slots S = 64 j + lane, i.e. logical pairs P = 64 j + (lane ^ h) with
+        // h = (4 (j & 3) + lane / 16) & 15: elements 128 j + eo, eo < 128.  The element offset inside
+        // the tile is U(j) | V(lane, j & 3): U(j) (uniform, from bits >= 7) and V (bits < 7, per lane).
+        uint64_t vlo[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t h = (4u * q + lane / 16u) & 15u;
+            const uint32_t eo = 2u * (lane ^ h);
+            vlo[q] = (lowb < 7 ? sbo[eo >> lowb] : 0) | (eo & lowmask);
+        }
+        const unsigned char *sbytes = reinterpret_cast<const unsigned char *>(st);
+        auto dma_tile = [&](unsigned char *buf, uint64_t tt) {
+            const unsigned char *tile = sbytes + tile_base(tt) * sizeof(C2<float>);
+#pragma unroll 16
+            for (uint32_t j = 0; j < NS / 64; ++j)
+                __builtin_amdgcn_global_load_lds(tile + (pd->dma_u[j] | vlo[j & 3u]) * sizeof(C2<float>),
+                                                 (lds_void_t *)(buf + j * 1024u), 16, 0, 0);
+        };
+        int cur = 0;
+        dma_tile(smem, t);
+        for (; t < ntiles; t += G) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            lds_barrier();
+            if (t + G < ntiles) dma_tile(smem + (cur ^ 1) * BUF, t + G);
+            for (int g = 0; g < ng; ++g) lds_barrier();
+            cur ^= 1;
+        }
+    } else {
+        // ------------------------------------------------ worker waves
+        const uint32_t w = tid - 64;
+        int cur = 0;
+        for (; t < ntiles; t += G) {
+            lds_barrier();
+            unsigned char *buf = smem + cur * BUF;
+            for (int g = 0; g < ng; ++g) {
+                const GroupDev &Gd = gd[g0 + g];
+                uint32_t boff[NR];
+                boff[0] = 0;
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const uint32_t bq = swz(1u << Gd.b[q]);
+#pragma unroll
+                    for (int r = (1 << q); r < (2 << q); ++r) boff[r] = boff[r - (1 << q)] ^ bq;
+                }
+                const int nops = Gd.nops, o0 = Gd.op0;
+#pragma unroll 1
+                for (uint32_t sl = w; sl < NSLOT; sl += NW) {
+                    uint32_t bidx = sl;
+#pragma unroll
+                    for (int q = 0; q < R; ++q) bidx = ins0_32(bidx, Gd.bs[q]);
+                    const uint32_t bb = swz(bidx);
+                    C2<float> a[NR];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) a[r] = *reinterpret_cast<const C2<float> *>(buf + (bb ^ boff[r]));
+#pragma unroll 1
+                    for (int o = 0; o < nops; ++o) {
+                        const OpDev<float> &op = od[o0 + o];
+                        const bool ts = (bidx & op.cmt) == op.cmt;
+                        apply_op<float, R>(a, op, ts);
+                    }
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) *reinterpret_cast<C2<float> *>(buf + (bb ^ boff[r])) = a[r];
+                }
+                lds_barrier();
+            }
+            const uint64_t tb = tile_base(t);
+            float *base = reinterpret_cast<float *>(st);
+#pragma unroll 4
+            for (uint32_t S = w; S < NS; S += NW) {
+                const uint32_t e = 2u * (S ^ ((S >> 4) & 15u));
+                const uint64_t gi = tb | sbo[e >> lowb] | (e & lowmask);
+                const float4 v = *reinterpret_cast<const float4 *>(buf + 16u * S);
+                *reinterpret_cast<float4 *>(base + 2 * gi) = v;
+            }
+            cur ^= 1;
+        }
+    }
+}
+
+// Persistent single-buffer variant: every wave loads (16-byte global_load_lds), computes and
+// stores each tile in turn; two workgroups share a CU so one computes while the other streams.
+template <int K, int R, int NT>
+__global__ __launch_bounds__(NT, 2) void k_pass_pers(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
+                                                  const GroupDev *__restrict__ gd,
+                                                  const OpDev<float> *__restrict__ od, uint64_t ntiles) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr uint32_t NE = 1u << K, NS = NE / 2, NJ = NS / 64, NWAVE = NT / 64;
+    constexpr int NR = 1 << R;
+    constexpr uint32_t NSLOT = NE >> R;
+    static_assert(NJ % NWAVE == 0 && (NWAVE & 3) == 0, "each wave issues every 4th DMA instruction class");
+    unsigned char *buf = smem;
+    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + NE * 8);
+    const int m = pd->m, lowb = K - m;
+    const uint32_t lowmask = (1u << lowb) - 1u;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    for (uint32_t q = tid; q < (1u << m); q += NT) {
+        uint64_t o = 0;
+        for (int i = 0; i < m; ++i)
+            if ((q >> i) & 1u) o |= uint64_t(1) << pd->h[i];
+        sbo[q] = o;
+    }
+    lds_barrier();
+    // this wave issues DMA instructions j = wave + NWAVE i, all with j & 3 == wave & 3
+    const uint32_t hq = (4u * (wave & 3u) + lane / 16u) & 15u;
+    const uint32_t eo = 2u * (lane ^ hq);
+    const uint64_t vlo = (lowb < 7 ? sbo[eo >> lowb] : 0) | (eo & lowmask);
+    const unsigned char *sbytes = reinterpret_cast<const unsigned char *>(st);
+    float *sf = reinterpret_cast<float *>(st);
+    const int ng = pd->ngroups, g0 = pd->group0;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint64_t tb = t << lowb;
+        for (int i = 0; i < m; ++i) tb = ins0_64(tb, pd->hs[i]);
+        const unsigned char *tile = sbytes + tb * sizeof(C2<float>);
+#pragma unroll
+        for (uint32_t i = 0; i < NJ / NWAVE; ++i) {
+            const uint32_t j = wave + NWAVE * i;
+            __builtin_amdgcn_global_load_lds(tile + (pd->dma_u[j] | vlo) * sizeof(C2<float>),
+                                             (lds_void_t *)(buf + j * 1024u), 16, 0, 0);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        lds_barrier();
+        for (int g = 0; g < ng; ++g) {
+            const GroupDev &Gd = gd[g0 + g];
+            uint32_t boff[NR];
+            boff[0] = 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const uint32_t bq = swz(1u << Gd.b[q]);
+#pragma unroll
+                for (int r = (1 << q); r < (2 << q); ++r) boff[r] = boff[r - (1 << q)] ^ bq;
+            }
+            const int nops = Gd.nops, o0 = Gd.op0;
+#pragma unroll 1
+            for (uint32_t sl = tid; sl < NSLOT; sl += NT) {
+                uint32_t bidx = sl;
+#pragma unroll
+                for (int q = 0; q < R; ++q) bidx = ins0_32(bidx, Gd.bs[q]);
+                const uint32_t bb = swz(bidx);
+                C2<float> a[NR];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) a[r] = *reinterpret_cast<const C2<float> *>(buf + (bb ^ boff[r]));
+#pragma unroll 1
+                for (int o = 0; o < nops; ++o) {
+                    const OpDev<float> &op = od[o0 + o];
+                    const bool ts = (bidx & op.cmt) == op.cmt;
+                    apply_op<float, R>(a, op, ts);
+                }
+#pragma unroll
+                for (int r = 0; r < NR; ++r) *reinterpret_cast<C2<float> *>(buf + (bb ^ boff[r])) = a[r];
+            }
+            lds_barrier();
+        }
+#pragma unroll 4
+        for (uint32_t S = tid; S < NS; S += NT) {
+            const uint32_t e = 2u * (S ^ ((S >> 4) & 15u));
+            const uint64_t gi = tb | sbo[e >> lowb] | (e & lowmask);
+            const float4 v = *reinterpret_cast<const float4 *>(buf + 16u * S);
+            *reinterpret_cast<float4 *>(sf + 2 * gi) = v;
+        }
+        lds_barrier();
+    }
+}
+
+size_t pass_pers_lds_bytes(int K, int m) {
+    return (size_t(1) << K) * 8 + (size_t(1) << m) * sizeof(uint64_t);
+}
+
+hipError_t launch_pass_pers(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+                            uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s) {
+    C2<float> *p = reinterpret_cast<C2<float> *>(st);
+    switch (K) {
+    case 12: hipLaunchKernelGGL((k_pass_pers<12, 4, 256>), dim3(grid), dim3(256), lds, s, p, pd, gd, od, ntiles); break;
+    case 13: hipLaunchKernelGGL((k_pass_pers<13, 5, 256>), dim3(grid), dim3(256), lds, s, p, pd, gd, od, ntiles); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t pass_dma_lds_bytes(int K, int m) {
+    return (size_t(2) << K) * 8 + (size_t(1) << m) * sizeof(uint64_t);
+}
+
+hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+                           uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s) {
+    C2<float> *p = reinterpret_cast<C2<float> *>(st);
+    switch (K) {
+    case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, ntiles); break;
+    case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, ntiles); break;
+    case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, ntiles); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // One workgroup runs a whole small state (<= 2^kSmallMaxBits amplitudes), op by op, directly
@@ -699,6 +1004,8 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<float, 13>)) SVX_ATTR((k_pass<float, 14>))
     SVX_ATTR((k_pass<double, 10>)) SVX_ATTR((k_pass<double, 11>)) SVX_ATTR((k_pass<double, 12>))
     SVX_ATTR((k_pass<double, 13>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
+    SVX_ATTR((k_pass_dma<12, 4, 256>)) SVX_ATTR((k_pass_dma<13, 4, 512>)) SVX_ATTR((k_pass_dma<13, 5, 256>))
+    SVX_ATTR((k_pass_pers<12, 4, 256>)) SVX_ATTR((k_pass_pers<13, 5, 256>))
 #undef SVX_ATTR
     return e;
 }

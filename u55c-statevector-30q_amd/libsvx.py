@@ -19,7 +19,7 @@ C64, C128 = 0, 1
 SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -1, -2, -3, -4, -5, -6
 
 SYMBOLS = ["svx_init", "svx_run_segment", "svx_engine", "svx_prepare", "svx_run_prepared",
-           "svx_prepared_info", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
+           "svx_prepared_info", "svx_prepared_kernel", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
            "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync", "svx_last_error",
            "svx_version"]
 
@@ -44,6 +44,7 @@ _SIGS = {
     "svx_prepare": ([_PI32, _I64, _I32, _I32, ctypes.POINTER(_I64)], _I32),
     "svx_run_prepared": ([_VP, _I64, _VP], _I32),
     "svx_prepared_info": ([_I64, _PI32, _PI32], _I32),
+    "svx_prepared_kernel": ([_I64, _I32, ctypes.c_char_p, _I32], _I32),
     "svx_permute": ([_VP, _VP, _I32, _PI32, _I32, _VP], _I32),
     "svx_swap_words": ([_VP, _VP, _I64, _I64, _I64, _VP], _I32),
     "svx_zero_state": ([_VP, _I32, _I32, _VP], _I32),
@@ -164,6 +165,13 @@ def prepared_info(handle):
     a, b = ctypes.c_int32(0), ctypes.c_int32(0)
     _check(lib.svx_prepared_info(handle, ctypes.byref(a), ctypes.byref(b)))
     return a.value, b.value
+
+
+def prepared_kernel(handle, pass_index):
+    lib = load()
+    buf = ctypes.create_string_buffer(64)
+    _check(lib.svx_prepared_kernel(handle, pass_index, buf, 64))
+    return buf.value.decode()
 
 
 def permute(dst, src, perm, stream=None):
