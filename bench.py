@@ -61,6 +61,67 @@ def cpu_baseline(max_seconds=30.0):
                         f"pinned to core {core} of {os.cpu_count()}"))
 
 
+def cp_ladder_section(libsvx, circuit, compiler, torch, np, n=30, seed=300):
+    """Controlled-phase ladder (H(j), CP(pi/2^(k-j)) control k > j, j = 0..n-1) on a seeded random
+    product state, complex64 and complex128 runs of the same records, error growth per pass, then
+    the bit reversal perm[i] = n-1-i as one permutation sweep (checked bit-exact by reversing back)."""
+    qc = circuit.cp_ladder(n)
+    prog, fmap, st = compiler.compile_gpu(qc)
+    assert fmap == list(range(n))
+    th, ph = circuit.product_state_angles(n, seed)
+    amps = np.stack([np.cos(th / 2), np.exp(1j * ph) * np.sin(th / 2)], axis=1)
+    s64 = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
+    s128 = torch.empty(1 << n, dtype=torch.complex128, device="cuda:0")
+    single = [seg for _, w in prog for seg in compiler.split_passes(w)]
+    h64 = [libsvx.prepare(w, libsvx.C64, n) for w in single]
+    h128 = [libsvx.prepare(w, libsvx.C128, n) for w in single]
+    growth = []
+    libsvx.product_state(s64, amps)
+    libsvx.product_state(s128, amps)
+    for i, (a, b) in enumerate(zip(h64, h128)):
+        libsvx.run_prepared(s64, a)
+        libsvx.run_prepared(s128, b)
+        r = libsvx.fidelity_f64(s128, s64)
+        growth.append([i + 1, r["one_minus_f"], r["max_abs_err"]])
+    stream = torch.cuda.current_stream()
+    times = {}
+    for name, s, hs in (("c64", s64, h64), ("c128", s128, h128)):
+        libsvx.product_state(s, amps)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for h in hs:
+            libsvx.run_prepared(s, h)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1)
+    rev = list(range(n))[::-1]
+    t64 = torch.empty_like(s64)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    libsvx.permute(t64, s64, rev)
+    e0.record(stream)
+    libsvx.permute(t64, s64, rev)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    perm_ms = e0.elapsed_time(e1)
+    back = torch.empty_like(s64)
+    libsvx.permute(back, t64, rev)
+    exact = bool(torch.equal(back, s64))
+    t128 = torch.empty_like(s128)
+    libsvx.permute(t128, s128, rev)
+    final = libsvx.fidelity_f64(t128, t64)
+    del s64, s128, t64, t128, back
+    torch.cuda.empty_cache()
+    nb = 2 * (1 << n) * 8
+    return {"workload": f"cp ladder, {n} qubits: {sum(g.name == 'h' for g in qc.data)} H + "
+                        f"{sum(g.name == 'cp' for g in qc.data)} CP, product state seed {seed}, then bit reversal",
+            "passes": len(single), "c64_ms": round(times["c64"], 3), "c128_ms": round(times["c128"], 3),
+            "c64_pass_gbs": round(len(single) * nb / (times["c64"] * 1e-3) / 1e9, 1),
+            "bitrev_ms": round(perm_ms, 3), "bitrev_gbs": round(nb / (perm_ms * 1e-3) / 1e9, 1),
+            "bitrev_roundtrip_bit_exact": exact,
+            "final_vs_c128": final,
+            "error_growth_pass_1mF_maxabs": [[g[0], float(f"{g[1]:.3e}"), float(f"{g[2]:.3e}")] for g in growth]}
+
+
 def _run_cpu_child():
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"],
                        capture_output=True, text=True, timeout=600)
@@ -153,6 +214,7 @@ def main():
         del s64, s128
         torch.cuda.empty_cache()
 
+    ladder = None if a.no_judge else cp_ladder_section(libsvx, circuit, compiler, torch, np)
     cpu = None if a.no_cpu else _run_cpu_child()
     out = {
         "metric": "30q c64 pass GB/s (% of 8 TB/s)",
@@ -183,6 +245,7 @@ def main():
                          "n_at_or_above_70pct": int(sum(g >= 0.7 * PEAK_GBS for g in per_pass_gbs)),
                          "n": len(per_pass_gbs)},
         "judge_vs_c128": judge,
+        "cp_ladder_30q": ladder,
         "cpu_baseline": cpu,
         "device": caps.arch.decode(),
     }

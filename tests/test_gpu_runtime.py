@@ -93,3 +93,31 @@ def test_gpu30_workload_full_size(gpu):
     assert r["one_minus_f"] < 1e-6 and r["fidelity"] > 1 - 1e-6, r
     del s64, s128
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_cp_ladder_product_state_and_bit_reversal(gpu, n):
+    """Controlled-phase ladder on a seeded product state (c64 and c128) against the golden model,
+    then the bit reversal as one permutation sweep, bit-exact against np.transpose."""
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    qc = C.cp_ladder(n)
+    prog, fmap, st = K.compile_gpu(qc)
+    th, ph = C.product_state_angles(n, 300)
+    amps = np.stack([np.cos(th / 2), np.exp(1j * ph) * np.sin(th / 2)], axis=1)
+    s0 = np.ones(1, dtype=np.complex128)
+    for q in range(n):
+        s0 = np.kron(amps[q], s0)
+    gold = model.run_program(prog, n, 1, fmap, state=s0.copy(), ctype=np.complex128)
+    rev = list(range(n))[::-1]
+    for dt, tol in ((torch.complex128, 1e-12), (torch.complex64, 2e-6)):
+        s = torch.empty(1 << n, dtype=dt, device="cuda")
+        libsvx.product_state(s, amps)
+        for _, w in prog:
+            libsvx.run_segment(s, w)
+        t = torch.empty_like(s)
+        libsvx.permute(t, s, rev)
+        torch.cuda.synchronize()
+        got, before = t.cpu().numpy(), s.cpu().numpy()
+        assert np.array_equal(got, model.remap(before, n, rev))
+        assert np.max(np.abs(before - gold)) < tol, dt

@@ -298,6 +298,43 @@ def schedule(ops, n, L, k, stats, remap_policy="lru", tiling=True, l2p0=None, M=
     return steps, l2p
 
 
+def _is_diagonal(op):
+    """True when the op is diagonal in the computational basis (a MUX2 whose M0 and M1 are both
+    diagonal); diagonal ops commute with each other even when they share qubits."""
+    if not hasattr(op, "M0"):
+        return False
+    M0, M1 = np.asarray(op.M0), np.asarray(op.M1)
+    return M0[0, 1] == 0 and M0[1, 0] == 0 and M1[0, 1] == 0 and M1[1, 0] == 0
+
+
+def _dependency_graph(ops):
+    """Edges i -> j (i < j) when ops i and j share a qubit and do not both act diagonally."""
+    nq = len(ops)
+    last_nd = {}           # qubit -> last non-diagonal op
+    diag_since = {}        # qubit -> diagonal ops after it
+    preds = [0] * nq
+    succs = [[] for _ in range(nq)]
+    for i, op in enumerate(ops):
+        deps = set()
+        diag = _is_diagonal(op)
+        for q in op.q:
+            if q in last_nd:
+                deps.add(last_nd[q])
+            if not diag:
+                deps.update(diag_since.get(q, ()))
+        for q in op.q:
+            if diag:
+                diag_since.setdefault(q, []).append(i)
+            else:
+                last_nd[q] = i
+                diag_since[q] = []
+        deps.discard(i)
+        preds[i] = len(deps)
+        for d in deps:
+            succs[d].append(i)
+    return preds, succs
+
+
 def schedule_window(ops, n, k, stats, M, max_ops=GPU_MAX_OPS, max_cost=None):
     """MI355X pass packer for one slice (L = n): list scheduling over the op dependency DAG.
 
@@ -308,18 +345,7 @@ def schedule_window(ops, n, k, stats, M, max_ops=GPU_MAX_OPS, max_cost=None):
     window are not handled here (the caller falls back to the reference scheduler).
     """
     nq = len(ops)
-    last = {}
-    preds = [0] * nq
-    succs = [[] for _ in range(nq)]
-    for i, op in enumerate(ops):
-        deps = set()
-        for q in op.q:
-            if q in last:
-                deps.add(last[q])
-            last[q] = i
-        preds[i] = len(deps)
-        for d in deps:
-            succs[d].append(i)
+    preds, succs = _dependency_graph(ops)
     ready = sorted(i for i in range(nq) if preds[i] == 0)
     passes = []
     done = 0

@@ -715,7 +715,7 @@ int svx_permute(void *dst, const void *src, int32_t n, const int32_t *perm, int3
         }
     }
     const int v = n <= 12 ? n : 12;
-    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 5)) * (dtype == SVX_C64 ? 8 : 16);
+    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 4)) * (dtype == SVX_C64 ? 8 : 16);
     hipError_t e = launch_permute(dtype, dst, src, (const PermDev *)pdv, grid, lds, (hipStream_t)stream);
     if (e != hipSuccess) return hipfail(e, "permute launch");
     return 0;

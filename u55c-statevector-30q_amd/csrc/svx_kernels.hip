@@ -722,6 +722,26 @@ __global__ __launch_bounds__(256) void k_permute(C2<T> *__restrict__ dst, const 
         }
     }
     const uint32_t NE = 1u << v, cm = (1u << c) - 1u;
+    if (sizeof(T) == 4 && c >= 1) {
+        // complex64: adjacent element pairs (16 bytes) on both sides; LDS pads 2 elements per 32 so
+        // new-order pairs stay 16-byte aligned
+        auto lp = [](uint32_t e) { return e + ((e >> 5) << 1); };
+        const uint32_t b0 = uint32_t(pd->rdlo[1]);    // LDS position bit of old bit 0
+        for (uint32_t q = 2 * threadIdx.x; q < NE; q += 512) {
+            const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
+            const float4 v2 = *reinterpret_cast<const float4 *>(
+                reinterpret_cast<const float *>(src) + 2 * (ob | (q & cm) | pd->oldhi[q >> c]));
+            tile[lp(pos)] = C2<T>{T(v2.x), T(v2.y)};
+            tile[lp(pos | b0)] = C2<T>{T(v2.z), T(v2.w)};
+        }
+        __syncthreads();
+        for (uint32_t l = 2 * threadIdx.x; l < NE; l += 512) {
+            const C2<T> a0 = tile[lp(l)], a1 = tile[lp(l) + 1];
+            *reinterpret_cast<float4 *>(reinterpret_cast<float *>(dst) + 2 * (nb | (l & cm) | pd->newhi[l >> c])) =
+                make_float4(float(a0.x), float(a0.y), float(a1.x), float(a1.y));
+        }
+        return;
+    }
     for (uint32_t q = threadIdx.x; q < NE; q += 256) {
         const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
         tile[lpad(pos)] = src[ob | (q & cm) | pd->oldhi[q >> c]];
