@@ -132,6 +132,29 @@ def test_compile_gpu_streams_vs_dense(n):
                 assert 1 - judge.fidelity(ref, out) < 1e-10, (qc.name, k, sched)
 
 
+@pytest.mark.parametrize("n", [6, 9])
+def test_compile_fused_vs_dense(n):
+    """Fused form (OP_DENSE blocks padded to the fusion width + leftover MUX2 ops) through the
+    golden model against the gate-level simulator; records carry float32 matrices."""
+    for qc in [zoo(n, 7), C.make("qft", n), C.make("rqc", n), C.u3_cx_layers(n, 8, 4)]:
+        ref = dense.simulate(qc.tuples(), n)
+        for width in (4, 5):
+            for k in (max(width, 6), n):
+                prog, fmap, st = K.compile_fused(qc, k=k, width=width)
+                out = model.run_program(prog, n, 1, fmap, ctype=np.complex128)
+                assert 1 - judge.fidelity(ref, out) < 1e-10, (qc.name, k, width)
+                if qc.name.startswith("u3cx"):
+                    assert st.n_dense > 0
+
+
+def test_fused_form_reduces_passes_at_30q():
+    """30 qubits, depth 20: fusing into 5-qubit dense blocks needs fewer sweeps than plain MUX2."""
+    qc = C.workload_gpu30()
+    _, _, plain = K.compile_gpu(qc)
+    _, _, fused = K.compile_fused(qc, width=5)
+    assert fused.n_dense > 0 and fused.n_passes < plain.n_passes
+
+
 def test_encode_decode_roundtrip():
     rng = np.random.default_rng(0)
     qc = zoo(8, 9)

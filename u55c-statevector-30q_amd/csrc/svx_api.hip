@@ -111,7 +111,11 @@ struct RawOp {
     int t;
     uint32_t cmask;
     float f[16];
+    int dense_f = 0;           // fused-form record: number of qubits (0 for MUX2)
+    int dq[5] = {0, 0, 0, 0, 0};
+    std::vector<float> dm;     // 2^f x 2^f complex64, row-major (re, im)
 };
+constexpr int kOpDense = 2;
 struct RawPass {
     int m;
     int h[kMaxM];
@@ -158,10 +162,32 @@ int parse_segment(const int32_t *w, int64_t nw, RawSeg *out) {
                 if (rp.h[j] == rp.h[i]) return fail(SVX_E_FORMAT, "repeated high qubit in a pass");
         }
         if (nops < 0) return fail(SVX_E_FORMAT, "negative op count");
-        if (pos + (int64_t)nops * kOpW > nw) return fail(SVX_E_FORMAT, "QSV2 stream truncated in an op record");
         rp.ops.resize(nops);
         for (int o = 0; o < nops; ++o) {
             const int32_t *r = w + pos;
+            if (pos + 8 > nw) return fail(SVX_E_FORMAT, "QSV2 stream truncated in an op record");
+            if (r[0] == kOpDense) {
+                // fused form: [OP_DENSE, f, q0..q4, 0] + 2 * 4^f float32 words
+                const int f = r[1];
+                if (f < 1 || f > 5) return fail(SVX_E_FORMAT, "dense record with f outside 1..5");
+                const int64_t len = 8 + (int64_t(2) << (2 * f));
+                if (pos + len > nw) return fail(SVX_E_FORMAT, "QSV2 stream truncated in a dense record");
+                RawOp &ro = rp.ops[o];
+                ro.dense_f = f;
+                for (int i = 0; i < f; ++i) {
+                    ro.dq[i] = r[2 + i];
+                    if (ro.dq[i] < 0 || ro.dq[i] >= k) return fail(SVX_E_FORMAT, "dense op qubit outside the tile");
+                    for (int i2 = 0; i2 < i; ++i2)
+                        if (ro.dq[i2] == ro.dq[i]) return fail(SVX_E_FORMAT, "repeated dense op qubit");
+                }
+                ro.dm.resize(size_t(2) << (2 * f));
+                std::memcpy(ro.dm.data(), r + 8, ro.dm.size() * sizeof(float));
+                ro.t = ro.dq[0];
+                ro.cmask = 0;
+                pos += len;
+                continue;
+            }
+            if (pos + kOpW > nw) return fail(SVX_E_FORMAT, "QSV2 stream truncated in an op record");
             if (r[0] != kOpMux2) return fail(SVX_E_FORMAT, "unknown opcode " + std::to_string(r[0]));
             if (r[1] < 0 || r[1] >= k) return fail(SVX_E_FORMAT, "op target outside the tile");
             if ((uint32_t)r[2] >> k) return fail(SVX_E_FORMAT, "op control mask outside the tile");
@@ -200,6 +226,7 @@ struct LaunchInfo {
     size_t lds;
     int64_t pass_index;   // index into the PassDev array
     bool empty;           // no op changes the state (skip launch)
+    bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
 };
 
 struct Prepared {
@@ -207,7 +234,7 @@ struct Prepared {
     std::vector<int32_t> words;
     std::string pick;           // SVX_PASS_KERNEL at preparation time
     void *dev = nullptr;        // [PassDev x P][GroupDev x G][OpDev<T> x O]
-    size_t pass_off = 0, group_off = 0, op_off = 0;
+    size_t pass_off = 0, group_off = 0, op_off = 0, w_off = 0;
     std::vector<LaunchInfo> launches;
     int device = 0;
 };
@@ -261,11 +288,27 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     std::vector<GroupDev> groups;
     std::vector<OpDev<T>> ops;
     std::vector<LaunchInfo> launches;
+    std::vector<float> wmat;      // dense matrices, MFMA A-operand order
     for (const RawPass &rp : seg.passes) {
         // classify
         std::vector<OpDev<T>> pops;
+        std::vector<const RawOp *> dense_of;   // parallel to pops: the dense record, or null
         double cost = 0;
+        bool has_dense = false;
         for (const RawOp &ro : rp.ops) {
+            if (ro.dense_f) {
+                if (dtype != SVX_C64)
+                    return fail(SVX_E_ARG, "dense (fused-form) records run in complex64; complex128 runs use the plain MUX2 records");
+                OpDev<T> od{};
+                od.kind = -1;
+                od.t = ro.dq[0];
+                pops.push_back(od);
+                dense_of.push_back(&ro);
+                has_dense = true;
+                cost += 8.0 * (ro.dense_f <= 4 ? 16 : 32) / 4.0;
+                continue;
+            }
+            dense_of.push_back(nullptr);
             OpDev<T> od{};
             for (int e = 0; e < 8; ++e) {
                 od.m0[e] = T(ro.f[e]);
@@ -280,7 +323,10 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 continue;
             }
             if (unc) {
-                if (is_id(od.m0)) continue;   // exact no-op
+                if (is_id(od.m0)) {   // exact no-op
+                    dense_of.pop_back();
+                    continue;
+                }
                 od.kind = is_x(od.m0) ? K_XSWAP : (is_diag(od.m0) ? K_DIAG : K_FULL);
             } else if (is_id(od.m0)) {
                 if (is_x(od.m1)) od.kind = K_CXSWAP;
@@ -296,7 +342,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         }
         int mode;   // 0 small, 1 single-buffer, 2 DMA, 3 persistent
         if (small) mode = 0;
-        else if (pick == "v1" || !dma_ok) mode = 1;
+        else if (pick == "v1" || !dma_ok || has_dense) mode = 1;
         else if (pick == "pers") mode = 3;
         else if (pick == "dma" || pick == "dma5") mode = 2;
         else mode = cost <= kDmaMaxCost ? 2 : 1;
@@ -327,6 +373,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         pd.nops = (int)pops.size();
         LaunchInfo li{};
         li.small = mode == 0;
+        li.dense = has_dense;
         li.dma = mode == 2;
         li.pers = mode == 3;
         li.dma_variant = dma5 ? 100 : 0;
@@ -350,17 +397,71 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         }
         launches.push_back(li);
         if (mode == 0) {
-            for (auto &o : pops) ops.push_back(o);
+            for (size_t i = 0; i < pops.size(); ++i) {
+                OpDev<T> o = pops[i];
+                if (dense_of[i]) {   // raw matrix for the small-state kernel's direct product
+                    const RawOp &ro = *dense_of[i];
+                    GroupDev gd{};
+                    gd.type = 2;
+                    gd.f = ro.dense_f;
+                    for (int q = 0; q < ro.dense_f; ++q) gd.dq[q] = gd.dqs[q] = ro.dq[q];
+                    gd.w_off = (int64_t)wmat.size();
+                    wmat.insert(wmat.end(), ro.dm.begin(), ro.dm.end());
+                    o.kind = K_DENSE;
+                    o.cmask = (int32_t)groups.size();
+                    groups.push_back(gd);
+                }
+                ops.push_back(o);
+            }
             pd.ngroups = 0;
             passes.push_back(pd);
             continue;
         }
-        // groups: maximal runs of ops whose targets fit R register bits
+        // groups: maximal runs of ops whose targets fit R register bits; a dense op is a group of its own
         size_t i = 0;
         while (i < pops.size()) {
+            if (dense_of[i]) {
+                const RawOp &ro = *dense_of[i];
+                const int f0 = ro.dense_f, F = std::max(4, f0);
+                GroupDev gd{};
+                gd.type = 1;
+                gd.f = F;
+                std::vector<int> qs(ro.dq, ro.dq + f0);
+                for (int b = 0; (int)qs.size() < F && b < K; ++b)
+                    if (std::find(qs.begin(), qs.end(), b) == qs.end()) qs.push_back(b);
+                if ((int)qs.size() < F) return fail(SVX_E_CAPACITY, "tile too small for a dense op");
+                for (int q = 0; q < F; ++q) gd.dq[q] = gd.dqs[q] = qs[q];
+                std::sort(gd.dqs, gd.dqs + F);
+                // padded matrix: the extra qubits are the high index bits (identity on them)
+                const int D = 1 << F, d0 = 1 << f0;
+                auto U = [&](int r, int c, int part) -> float {
+                    if ((r >> f0) != (c >> f0)) return 0.f;
+                    return ro.dm[2 * ((r & (d0 - 1)) * d0 + (c & (d0 - 1))) + part];
+                };
+                // W = [[Ur, -Ui], [Ui, Ur]] in A-operand order: wd[(rt * 2^F + s) * 64 + l] = W[32 rt + l % 32][2 s + l / 32]
+                const int NRT = F == 5 ? 2 : 1;
+                gd.w_off = (int64_t)wmat.size();
+                for (int rt = 0; rt < NRT; ++rt)
+                    for (int st = 0; st < D; ++st)
+                        for (int l = 0; l < 64; ++l) {
+                            const int row = 32 * rt + l % 32, col = 2 * st + l / 32;
+                            const int r = row % D, c = col % D;
+                            const bool rim = row >= D, cim = col >= D;
+                            float v;
+                            if (!rim && !cim) v = U(r, c, 0);
+                            else if (!rim && cim) v = -U(r, c, 1);
+                            else if (rim && !cim) v = U(r, c, 1);
+                            else v = U(r, c, 0);
+                            wmat.push_back(v);
+                        }
+                groups.push_back(gd);
+                ++i;
+                continue;
+            }
             std::vector<int> tg;
             size_t j = i;
             for (; j < pops.size(); ++j) {
+                if (dense_of[j]) break;
                 const int t = pops[j].t;
                 if (std::find(tg.begin(), tg.end(), t) == tg.end()) {
                     if ((int)tg.size() == R) break;
@@ -410,11 +511,13 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     pr->pass_off = 0;
     pr->group_off = al(passes.size() * sizeof(PassDev));
     pr->op_off = pr->group_off + al(groups.size() * sizeof(GroupDev));
-    const size_t total = std::max<size_t>(256, pr->op_off + al(ops.size() * sizeof(OpDev<T>)));
+    pr->w_off = pr->op_off + al(ops.size() * sizeof(OpDev<T>));
+    const size_t total = std::max<size_t>(256, pr->w_off + al(wmat.size() * sizeof(float)));
     std::vector<unsigned char> host(total, 0);
     if (!passes.empty()) std::memcpy(host.data() + pr->pass_off, passes.data(), passes.size() * sizeof(PassDev));
     if (!groups.empty()) std::memcpy(host.data() + pr->group_off, groups.data(), groups.size() * sizeof(GroupDev));
     if (!ops.empty()) std::memcpy(host.data() + pr->op_off, ops.data(), ops.size() * sizeof(OpDev<T>));
+    if (!wmat.empty()) std::memcpy(host.data() + pr->w_off, wmat.data(), wmat.size() * sizeof(float));
     HIPCHK(hipMalloc(&pr->dev, total));
     HIPCHK(hipMemcpy(pr->dev, host.data(), total, hipMemcpyHostToDevice));
     g_prep_bytes += total;
@@ -481,7 +584,9 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
                                 reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
         else if (pr->dtype == SVX_C64)
             e = launch_pass_c64(li.K, li.small, state, pd + li.pass_index, gd,
-                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.grid, li.lds, s);
+                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
+                                li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr,
+                                li.grid, li.lds, s);
         else
             e = launch_pass_c128(li.K, li.small, state, pd + li.pass_index, gd,
                                  reinterpret_cast<const OpDev<double> *>(base + pr->op_off), li.grid, li.lds, s);
@@ -627,7 +732,7 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
     else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
     else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5>" : ",4>");
-    else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + ">";
+    else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());
     return 0;
 }
@@ -676,8 +781,13 @@ int svx_engine(void *st, int64_t ins_off, void *stream) {
     for (int p = 0; p < hdr[2]; ++p) {
         int32_t ph[kPassW];
         HIPCHK(hipMemcpy(ph, ins + pos * 4, sizeof(ph), hipMemcpyDeviceToHost));
-        int64_t len = kPassW + (int64_t)std::max(0, ph[3]) * kOpW + (ph[0] > 4 ? kPassW : 0);
-        if (len > (int64_t(4) << 20)) return fail(SVX_E_FORMAT, "pass longer than the 4 MiB instruction region");
+        int64_t len = kPassW + (ph[0] > 4 ? kPassW : 0);
+        for (int o = 0; o < std::max(0, ph[3]); ++o) {
+            int32_t oh[2];
+            HIPCHK(hipMemcpy(oh, ins + (pos + len) * 4, sizeof(oh), hipMemcpyDeviceToHost));
+            len += oh[0] == kOpDense ? 8 + (int64_t(2) << (2 * std::min(5, std::max(1, oh[1])))) : kOpW;
+            if (len > (int64_t(1) << 20)) return fail(SVX_E_FORMAT, "pass longer than the 4 MiB instruction region");
+        }
         const size_t old = words.size();
         words.resize(old + len);
         HIPCHK(hipMemcpy(words.data() + old, ins + pos * 4, len * 4, hipMemcpyDeviceToHost));

@@ -27,6 +27,7 @@ enum OpKind : int32_t {
     K_MUX = 7,     // controlled, general: selected pairs M1, others M0
     K_CXSWAP1 = 8, // controlled exact X, one control in the register group, none outside
     K_CPHASE1 = 9, // controlled diag(1, e), one control in the register group, none outside
+    K_DENSE = 10,  // small-state kernel only: dense op, cmask = index of its GroupDev (type 2)
 };
 
 struct PassDev {
@@ -41,7 +42,12 @@ struct GroupDev {
     int32_t nops, op0;
     int32_t b[kR];               // workgroup-tile bit held by register bit q
     int32_t bs[kR];              // b sorted ascending (slot deposit)
-    int32_t pad_[2];
+    int32_t type;                // 0: MUX2 register group; 1: dense op, MFMA; 2: dense op, small-state kernel
+    int32_t f;                   // dense: qubits of the op (type 1: 4 or 5 after padding)
+    int32_t dq[5];               // dense: workgroup-tile bit of the op's qubit i (matrix index bit i)
+    int32_t dqs[5];              // dq sorted ascending (column deposit)
+    int64_t w_off;               // dense: float offset of the matrix (type 1: MFMA A-operand order;
+                                 // type 2: row-major re, im)
 };
 
 template <typename T>
@@ -65,7 +71,7 @@ struct PermDev {
 
 // launchers (svx_kernels.hip)
 hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
-                           const OpDev<float> *od, uint32_t grid, size_t lds, hipStream_t s);
+                           const OpDev<float> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                             const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t set_kernel_attributes();

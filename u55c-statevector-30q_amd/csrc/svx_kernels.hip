@@ -304,10 +304,81 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
     }
 }
 
-template <typename T, int K>
-__global__ __launch_bounds__(kNT) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
+// ------------------------------------------------------------------------------------------ dense op
+// Fused form: a dense 2^F x 2^F complex op on workgroup-tile bits dq[0..F-1], applied to the LDS tile
+// with FP32 matrix cores.  Real form Y = W X with W = [[Ur, -Ui], [Ui, Ur]] (2^(F+1) square) and the
+// columns of X the 2^(K-F) groups of 2^F amplitudes (rows: Re x_0.., Im x_0..).  One wavefront
+// handles 32 columns per v_mfma_f32_32x32x2f32 chain: lane l owns column l % 32, half h = l / 32
+// feeds K-rows 2s + h; the accumulator lane layout of the 32x32 output is row i = 8 (v / 4) + 4 h +
+// v % 4 for accumulator v.  W arrives pre-arranged on the host in A-operand order:
+// wd[(rt * 2^F + s) * 64 + l] = W[32 rt + l % 32][2 s + l / 32].
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+template <int K, int F>
+__device__ __forceinline__ void dense_apply(C2<float> *tile, const GroupDev &G, const float *__restrict__ wd,
+                                            uint32_t tid) {
+    constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
+    constexpr int NS = 1 << F;                     // K-steps of 2 real rows
+    constexpr int NX = NS / 2;                     // complex inputs per lane
+    constexpr uint32_t NCOL = 1u << (K - F);
+    constexpr uint32_t NB = NCOL / 32;             // 32-column batches per tile
+    const uint32_t lane = tid & 63u, wave = tid >> 6, h = lane >> 5, j = lane & 31u;
+    const float *w = wd + G.w_off;
+    float wa[NRT][NS];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wa[rt][s] = w[(rt * NS + s) * 64 + lane];
+    uint32_t lq[F];
+#pragma unroll
+    for (int i = 0; i < F; ++i) lq[i] = lpad(1u << G.dq[i]);
+    // padded LDS offset of component c (pad is linear over disjoint bits)
+    auto coff = [&](uint32_t c) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+            if ((c >> i) & 1u) o += lq[i];
+        return o;
+    };
+    for (uint32_t b = wave; b < NB; b += kNT / 64) {
+        uint32_t base = b * 32u + j;
+#pragma unroll
+        for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
+        const uint32_t pb = lpad(base);
+        C2<float> x[NX];
+#pragma unroll
+        for (int s = 0; s < NX; ++s) x[s] = tile[pb + coff(2u * s) + (h ? lq[0] : 0u)];
+        f16v acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+            acc[rt] = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const float bv = s < NX ? x[s].x : x[s - NX].y;
+                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[rt][s], bv, acc[rt], 0, 0, 0);
+            }
+        }
+        // lane half h holds components c = 8 a + 4 h + e
+        if (F == 4) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
+                tile[pb + coff(c)] = C2<float>{acc[0][v], acc[0][v + 8]};
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
+                tile[pb + coff(c)] = C2<float>{acc[0][v], acc[NRT - 1][v]};
+            }
+        }
+    }
+}
+
+template <typename T, int K, bool DN>
+__global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                               const GroupDev *__restrict__ gd,
-                                              const OpDev<T> *__restrict__ od) {
+                                              const OpDev<T> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr uint32_t NE = 1u << K;
     constexpr uint32_t NEP = NE + (NE >> 5);
@@ -339,6 +410,14 @@ __global__ __launch_bounds__(kNT) void k_pass(C2<T> *__restrict__ st, const Pass
     const int ng = pd->ngroups, g0 = pd->group0;
     for (int g = 0; g < ng; ++g) {
         const GroupDev &G = gd[g0 + g];
+        if constexpr (DN && sizeof(T) == 4) {
+            if (G.type == 1) {
+                if (G.f == 5) dense_apply<K, 5>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
+                else dense_apply<K, 4>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
+                __syncthreads();
+                continue;
+            }
+        }
         uint32_t poff[NR];   // padded LDS offset of register r (disjoint bits: pad(a|b) = pad(a)+pad(b))
         poff[0] = 0;
 #pragma unroll
@@ -625,7 +704,8 @@ hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *g
 // from the MUX2 definition.  Also the independent cross-check of k_pass in the tests.
 template <typename T>
 __global__ __launch_bounds__(kNT) void k_pass_small(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
-                                                    const OpDev<T> *__restrict__ od) {
+                                                    const OpDev<T> *__restrict__ od,
+                                                    const GroupDev *__restrict__ gd, const float *__restrict__ wd) {
     __shared__ C2<T> tile[1u << kSmallMaxBits];
     const int k = pd->k, m = pd->m, nb = pd->nbits;
     const int lowb = k - m;
@@ -645,6 +725,43 @@ __global__ __launch_bounds__(kNT) void k_pass_small(C2<T> *__restrict__ st, cons
     const int nops = pd->nops, o0 = pd->op0;
     for (int o = 0; o < nops; ++o) {
         const OpDev<T> &op = od[o0 + o];
+        if constexpr (sizeof(T) == 4) {
+            if (op.kind == K_DENSE) {
+                // y[row] = sum_c U[row][c] x[c] over the op's qubits; NE <= 2 * kNT
+                const GroupDev &G = gd[op.cmask];
+                const int f = G.f;
+                const uint32_t D = 1u << f;
+                const float *U = wd + G.w_off;
+                uint32_t qm = 0;
+                for (int i = 0; i < f; ++i) qm |= 1u << G.dq[i];
+                C2<T> y[2];
+                for (int r = 0; r < 2; ++r) {
+                    const uint32_t e = tid + r * kNT;
+                    if (e >= NE) break;
+                    uint32_t row = 0;
+                    for (int i = 0; i < f; ++i) row |= ((e >> G.dq[i]) & 1u) << i;
+                    const uint32_t base = e & ~qm;
+                    float ar = 0.f, ai = 0.f;
+                    for (uint32_t c = 0; c < D; ++c) {
+                        uint32_t idx = base;
+                        for (int i = 0; i < f; ++i)
+                            if ((c >> i) & 1u) idx |= 1u << G.dq[i];
+                        const float ur = U[2 * (row * D + c)], ui = U[2 * (row * D + c) + 1];
+                        const C2<T> x = tile[idx];
+                        ar = fmaf(ur, x.x, fmaf(-ui, x.y, ar));
+                        ai = fmaf(ur, x.y, fmaf(ui, x.x, ai));
+                    }
+                    y[r] = C2<T>{ar, ai};
+                }
+                __syncthreads();
+                for (int r = 0; r < 2; ++r) {
+                    const uint32_t e = tid + r * kNT;
+                    if (e < NE) tile[e] = y[r];
+                }
+                __syncthreads();
+                continue;
+            }
+        }
         const int t = op.t;
         const uint32_t cm = uint32_t(op.cmask);
         for (uint32_t p = tid; p < NE / 2; p += kNT) {
@@ -671,20 +788,33 @@ size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits
 
 template <typename T>
 static hipError_t launch_pass_t(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
-                                const OpDev<T> *od, uint32_t grid, size_t lds, hipStream_t s) {
+                                const OpDev<T> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s) {
     C2<T> *p = reinterpret_cast<C2<T> *>(st);
     if (small) {
-        hipLaunchKernelGGL(k_pass_small<T>, dim3(1), dim3(kNT), 0, s, p, pd, od);
+        hipLaunchKernelGGL(k_pass_small<T>, dim3(1), dim3(kNT), 0, s, p, pd, od, gd, wd);
         return hipGetLastError();
     }
+    if constexpr (sizeof(T) == 4) {
+        if (wd) {   // pass with dense (fused-form) groups: the MFMA instance
+            switch (K) {
+            case 10: hipLaunchKernelGGL((k_pass<T, 10, true>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+            case 11: hipLaunchKernelGGL((k_pass<T, 11, true>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+            case 12: hipLaunchKernelGGL((k_pass<T, 12, true>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+            case 13: hipLaunchKernelGGL((k_pass<T, 13, true>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+            case 14: hipLaunchKernelGGL((k_pass<T, 14, true>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+            default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
+    }
     switch (K) {
-    case 10: hipLaunchKernelGGL((k_pass<T, 10>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
-    case 11: hipLaunchKernelGGL((k_pass<T, 11>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
-    case 12: hipLaunchKernelGGL((k_pass<T, 12>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
-    case 13: hipLaunchKernelGGL((k_pass<T, 13>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od); break;
+    case 10: hipLaunchKernelGGL((k_pass<T, 10, false>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+    case 11: hipLaunchKernelGGL((k_pass<T, 11, false>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+    case 12: hipLaunchKernelGGL((k_pass<T, 12, false>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
+    case 13: hipLaunchKernelGGL((k_pass<T, 13, false>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd); break;
     case 14:
         if constexpr (sizeof(T) == 4) {
-            hipLaunchKernelGGL((k_pass<T, 14>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od);
+            hipLaunchKernelGGL((k_pass<T, 14, false>), dim3(grid), dim3(kNT), lds, s, p, pd, gd, od, wd);
             break;
         } else {
             return hipErrorInvalidValue;
@@ -695,12 +825,12 @@ static hipError_t launch_pass_t(int K, bool small, void *st, const PassDev *pd, 
 }
 
 hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
-                           const OpDev<float> *od, uint32_t grid, size_t lds, hipStream_t s) {
-    return launch_pass_t<float>(K, small, st, pd, gd, od, grid, lds, s);
+                           const OpDev<float> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s) {
+    return launch_pass_t<float>(K, small, st, pd, gd, od, wd, grid, lds, s);
 }
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                             const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s) {
-    return launch_pass_t<double>(K, small, st, pd, gd, od, grid, lds, s);
+    return launch_pass_t<double>(K, small, st, pd, gd, od, nullptr, grid, lds, s);
 }
 
 // ------------------------------------------------------------------------------------------ permute
@@ -1020,10 +1150,12 @@ hipError_t set_kernel_attributes() {
 #define SVX_ATTR(fn)                                                                              \
     if (e == hipSuccess) e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),               \
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    SVX_ATTR((k_pass<float, 10>)) SVX_ATTR((k_pass<float, 11>)) SVX_ATTR((k_pass<float, 12>))
-    SVX_ATTR((k_pass<float, 13>)) SVX_ATTR((k_pass<float, 14>))
-    SVX_ATTR((k_pass<double, 10>)) SVX_ATTR((k_pass<double, 11>)) SVX_ATTR((k_pass<double, 12>))
-    SVX_ATTR((k_pass<double, 13>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
+    SVX_ATTR((k_pass<float, 10, false>)) SVX_ATTR((k_pass<float, 11, false>)) SVX_ATTR((k_pass<float, 12, false>))
+    SVX_ATTR((k_pass<float, 13, false>)) SVX_ATTR((k_pass<float, 14, false>))
+    SVX_ATTR((k_pass<float, 10, true>)) SVX_ATTR((k_pass<float, 11, true>)) SVX_ATTR((k_pass<float, 12, true>))
+    SVX_ATTR((k_pass<float, 13, true>)) SVX_ATTR((k_pass<float, 14, true>))
+    SVX_ATTR((k_pass<double, 10, false>)) SVX_ATTR((k_pass<double, 11, false>)) SVX_ATTR((k_pass<double, 12, false>))
+    SVX_ATTR((k_pass<double, 13, false>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
     SVX_ATTR((k_pass_dma<12, 4, 256>)) SVX_ATTR((k_pass_dma<13, 4, 512>)) SVX_ATTR((k_pass_dma<13, 5, 256>))
     SVX_ATTR((k_pass_pers<12, 4, 256>)) SVX_ATTR((k_pass_pers<13, 5, 256>))
 #undef SVX_ATTR
