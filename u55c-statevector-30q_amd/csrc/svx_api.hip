@@ -10,6 +10,8 @@
 // exact zeros and ones.
 #include <hip/hip_runtime.h>
 
+#include "svx_half.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -269,6 +271,10 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     // groups) otherwise.  SVX_PASS_KERNEL=v1|dma|dma5|pers|auto forces a choice.
     const char *env = std::getenv("SVX_PASS_KERNEL");
     const std::string pick = env ? env : "auto";
+    // dense records: split-FP16 matrix cores unless every entry is 0 / +-1 (then FP32 matrix cores,
+    // exact); SVX_DENSE_MATH=f32|f16 forces one of them
+    const char *denv = std::getenv("SVX_DENSE_MATH");
+    const std::string dense_math = denv ? denv : "auto";
     const bool dma_ok = dtype == SVX_C64 && (k == 12 || k == 13) && nbits >= k + 9;
     if ((pick == "dma" || pick == "pers" || pick == "dma5") && !dma_ok && !small)
         return fail(SVX_E_ARG, "SVX_PASS_KERNEL=" + pick + " needs complex64, k in {12, 13} and >= 2^(k+9) amplitudes");
@@ -424,7 +430,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 const RawOp &ro = *dense_of[i];
                 const int f0 = ro.dense_f, F = std::max(4, f0);
                 GroupDev gd{};
-                gd.type = 1;
                 gd.f = F;
                 std::vector<int> qs(ro.dq, ro.dq + f0);
                 for (int b = 0; (int)qs.size() < F && b < K; ++b)
@@ -438,22 +443,45 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     if ((r >> f0) != (c >> f0)) return 0.f;
                     return ro.dm[2 * ((r & (d0 - 1)) * d0 + (c & (d0 - 1))) + part];
                 };
-                // W = [[Ur, -Ui], [Ui, Ur]] in A-operand order: wd[(rt * 2^F + s) * 64 + l] = W[32 rt + l % 32][2 s + l / 32]
+                // real form W = [[Ur, -Ui], [Ui, Ur]] (2D x 2D) acting on [Re x; Im x]
+                auto W = [&](int row, int col) -> float {
+                    const int r = row % D, c = col % D;
+                    const bool rim = row >= D, cim = col >= D;
+                    if (rim == cim) return U(r, c, 0);
+                    return rim ? U(r, c, 1) : -U(r, c, 1);
+                };
+                // entries in {0, +-1}: the FP32 products are exact (permutations stay bit-exact)
+                bool exact = true;
+                for (float v : ro.dm) exact = exact && (v == 0.f || v == 1.f || v == -1.f);
                 const int NRT = F == 5 ? 2 : 1;
+                while (wmat.size() % 4) wmat.push_back(0.f);   // 16-byte aligned operand blocks
                 gd.w_off = (int64_t)wmat.size();
-                for (int rt = 0; rt < NRT; ++rt)
-                    for (int st = 0; st < D; ++st)
-                        for (int l = 0; l < 64; ++l) {
-                            const int row = 32 * rt + l % 32, col = 2 * st + l / 32;
-                            const int r = row % D, c = col % D;
-                            const bool rim = row >= D, cim = col >= D;
-                            float v;
-                            if (!rim && !cim) v = U(r, c, 0);
-                            else if (!rim && cim) v = -U(r, c, 1);
-                            else if (rim && !cim) v = U(r, c, 1);
-                            else v = U(r, c, 0);
-                            wmat.push_back(v);
-                        }
+                if (dense_math == "f32" || (dense_math != "f16" && exact)) {
+                    // FP32 A operand of v_mfma_f32_32x32x2f32: wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32]
+                    gd.type = 1;
+                    for (int rt = 0; rt < NRT; ++rt)
+                        for (int st = 0; st < D; ++st)
+                            for (int l = 0; l < 64; ++l) wmat.push_back(W(32 * rt + l % 32, 2 * st + l / 32));
+                } else {
+                    // split-FP16 A operands of v_mfma_f32_32x32x16_f16: 2^14 W = Wh + Wl; block
+                    // ((rt NKS + ks) 2 + part) holds lane l, entry e = W[32 rt + l % 32][16 ks + 8 (l / 32) + e]
+                    gd.type = 3;
+                    const int NKS = D / 8;
+                    std::vector<uint16_t> hb;
+                    hb.reserve((size_t)NRT * NKS * 2 * 512);
+                    for (int rt = 0; rt < NRT; ++rt)
+                        for (int ks = 0; ks < NKS; ++ks)
+                            for (int part = 0; part < 2; ++part)
+                                for (int l = 0; l < 64; ++l)
+                                    for (int e = 0; e < 8; ++e) {
+                                        const float ws = W(32 * rt + l % 32, 16 * ks + 8 * (l / 32) + e) * 16384.0f;
+                                        const uint16_t hh = f32_to_f16_bits(ws);
+                                        hb.push_back(part == 0 ? hh : f32_to_f16_bits(ws - f16_bits_to_f32(hh)));
+                                    }
+                    const size_t o = wmat.size();
+                    wmat.resize(o + hb.size() / 2);
+                    std::memcpy(wmat.data() + o, hb.data(), hb.size() * sizeof(uint16_t));
+                }
                 groups.push_back(gd);
                 ++i;
                 continue;
@@ -531,8 +559,10 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     DeviceState *ds = nullptr;
     int rc = get_ds(&ds);
     if (rc) return rc;
+    // the cache key includes the environment choices that change the descriptors
     const char *kenv = std::getenv("SVX_PASS_KERNEL");
-    const std::string pick = kenv ? kenv : "auto";
+    const char *denv = std::getenv("SVX_DENSE_MATH");
+    const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);

@@ -375,6 +375,103 @@ __device__ __forceinline__ void dense_apply(C2<float> *tile, const GroupDev &G, 
     }
 }
 
+// Dense op on split-FP16 matrix cores: W (pre-scaled by 2^14 and split W = Wh + Wl on the host)
+// times a column block x scaled by a per-column power of two s (max |x s| < 2^15) and split
+// x s = xh + xl in FP16 (the split is exact in FP32); y = (Wh xh + Wh xl + Wl xh) / (2^14 s) with
+// FP32 accumulation.  The dropped term and the two-piece representations bound the relative error
+// per product by about 2^-21.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B) l % 32
+// and K entries 8 (l / 32) + 0..7 of each 16-deep K step.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+template <int K, int F>
+__device__ __forceinline__ void dense_apply_h(C2<float> *tile, const GroupDev &G, const float *__restrict__ wd,
+                                              uint32_t tid) {
+    constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
+    constexpr int D = 1 << F;
+    constexpr int NG = D / 16;                     // groups of 8 components per lane half
+    constexpr int NKS = 2 * NG;                    // 16-deep K steps (real parts, then imaginary parts)
+    constexpr uint32_t NCOL = 1u << (K - F);
+    constexpr uint32_t NB = NCOL / 32;
+    const uint32_t lane = tid & 63u, wave = tid >> 6, h = lane >> 5, j = lane & 31u;
+    const half8 *w8 = reinterpret_cast<const half8 *>(wd + G.w_off);
+    half8 ah[NRT][NKS], al[NRT][NKS];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            ah[rt][ks] = w8[((rt * NKS + ks) * 2 + 0) * 64 + lane];
+            al[rt][ks] = w8[((rt * NKS + ks) * 2 + 1) * 64 + lane];
+        }
+    uint32_t lq[F];
+#pragma unroll
+    for (int i = 0; i < F; ++i) lq[i] = lpad(1u << G.dq[i]);
+    auto coff = [&](uint32_t c) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+            if ((c >> i) & 1u) o += lq[i];
+        return o;
+    };
+    for (uint32_t b = wave; b < NB; b += kNT / 64) {
+        uint32_t base = b * 32u + j;
+#pragma unroll
+        for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
+        const uint32_t pb = lpad(base);
+        C2<float> x[NG][8];
+        float mx = 0.f;
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                x[g][e] = tile[pb + coff(16u * g + 8u * h + e)];
+                mx = fmaxf(mx, fmaxf(fabsf(x[g][e].x), fabsf(x[g][e].y)));
+            }
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        int ex = 0;
+        frexpf(mx, &ex);                           // mx < 2^ex
+        const int sh = mx > 0.f ? 15 - ex : 0;
+        half8 bh[NKS], bl[NKS];
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float xr = ldexpf(x[g][e].x, sh), xi = ldexpf(x[g][e].y, sh);
+                const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+                bh[g][e] = hr;
+                bl[g][e] = (_Float16)(xr - (float)hr);
+                bh[NG + g][e] = hi;
+                bl[NG + g][e] = (_Float16)(xi - (float)hi);
+            }
+        f16v acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+            acc[rt] = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rt][ks], bh[ks], acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bl[ks], acc[rt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks)
+                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bh[ks], acc[rt], 0, 0, 0);
+        }
+        const int back = -(sh + 14);
+        if (F == 4) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
+                tile[pb + coff(c)] = C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[0][v + 8], back)};
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
+                tile[pb + coff(c)] = C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[NRT - 1][v], back)};
+            }
+        }
+    }
+}
+
 template <typename T, int K, bool DN>
 __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                               const GroupDev *__restrict__ gd,
@@ -414,6 +511,12 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
             if (G.type == 1) {
                 if (G.f == 5) dense_apply<K, 5>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
                 else dense_apply<K, 4>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
+                __syncthreads();
+                continue;
+            }
+            if (G.type == 3) {
+                if (G.f == 5) dense_apply_h<K, 5>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
+                else dense_apply_h<K, 4>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
                 __syncthreads();
                 continue;
             }
