@@ -230,8 +230,39 @@ def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m, monkeypatch):
     s0 = _rand_state(rng, n, np.complex64)
     ref = model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128)
     got = {}
+    monkeypatch.setenv("SVX_FUSE", "off")
     for pick in ("dma", "v1"):
         monkeypatch.setenv("SVX_PASS_KERNEL", pick)
         got[pick] = run_hip(s0, seg)
         _check_c64(ref, got[pick], (n, k, max_m, pick))
     assert np.array_equal(got["dma"], got["v1"])
+
+
+@pytest.mark.parametrize("fuse,cost", [("off", None), ("auto", None), ("on", None), ("on", "0,0,5"),
+                                       ("on", "0,0,3"), ("on", "0,0,1")])
+@pytest.mark.parametrize("n,k,max_m", [(12, 10, 4), (16, 12, 6), (20, 13, 7)])
+def test_pass_fusion_matches_oracle(gpu, n, k, max_m, fuse, cost, monkeypatch):
+    """Pass fusion (MUX2 ops of a pass clustered into dense units on the matrix cores): every
+    setting, from off to fusing every block of 1, 3 or 5 tile bits, matches the complex128 golden
+    model; ops with controls and several kinds of matrices, including controls wider than a block."""
+    monkeypatch.setenv("SVX_FUSE", fuse)
+    if cost:
+        monkeypatch.setenv("SVX_FUSE_COST", cost)
+    rng = np.random.default_rng(n * 31 + k + max_m)
+    seg = random_segment(rng, n, k, n_passes=3, max_ops=40, max_m=max_m, ctrl_p=0.12)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got = run_hip(s0.astype(np.complex64), seg)
+    _check_c64(ref, got, (n, k, max_m, fuse, cost))
+
+
+def test_pass_fusion_of_permutations_is_exact(gpu, monkeypatch):
+    """Blocks of exact X / controlled-X ops fuse into 0/1 matrices, which run on the exact FP32
+    matrix-core path: the result equals the c64 golden model bit for bit."""
+    monkeypatch.setenv("SVX_FUSE", "on")
+    monkeypatch.setenv("SVX_FUSE_COST", "0,0,5")
+    rng = np.random.default_rng(5)
+    seg = random_segment(rng, 16, 12, n_passes=3, max_ops=40, kinds=[("x", "x"), ("i", "x")], max_m=6)
+    s0 = _rand_state(rng, 16, np.complex64)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex64)
+    assert np.array_equal(run_hip(s0, seg), ref)

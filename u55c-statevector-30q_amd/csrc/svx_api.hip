@@ -21,6 +21,7 @@
 #include <memory>
 #include <functional>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -256,6 +257,172 @@ uint64_t fnv1a(const int32_t *w, int64_t n, uint64_t seed) {
     return h;
 }
 
+// A dense unit: 2^f x 2^f complex matrix (row-major, re/im doubles) on tile bits q[0..f-1]; matrix
+// index bit i <-> tile bit q[i].  From an OP_DENSE record or from pass fusion.
+struct DenseUnit {
+    int f = 0;
+    int q[5] = {0, 0, 0, 0, 0};
+    std::vector<double> u;
+};
+
+// VALU cost of one op per amplitude, in FMAs (a general 2x2 MUX2 = 8), by OpKind
+const double kOpCost[] = {8, 4, 1, 8, 4, 2, 2, 8, 2, 2};
+
+struct FuseCfg {
+    int width = 5;          // largest block (tile bits)
+    double cost4 = 8.0;     // cost of one dense op padded to 4 qubits, same units as kOpCost
+    double cost5 = 16.0;    // ... 5 qubits
+};
+
+// Dense matrix of one MUX2 op on the block's tile bits qs (ascending; local index bit i <-> qs[i]):
+// the pair (c0, c1) differing in the target bit gets M1 when the control mask is non-zero and all
+// its bits are set in c0 (host/sv/model.py:59-71), else M0.
+template <typename T>
+void mux_matrix(const OpDev<T> &op, const std::vector<int> &qs, std::vector<double> &m) {
+    const int f = (int)qs.size(), D = 1 << f;
+    m.assign(size_t(2) * D * D, 0.0);
+    int tp = 0;
+    for (int i = 0; i < f; ++i)
+        if (qs[i] == op.t) tp = i;
+    const uint32_t cm = (uint32_t)op.cmask;
+    for (int c = 0; c < D; ++c) {
+        const int tb = (c >> tp) & 1;
+        const int c0 = c & ~(1 << tp), c1 = c | (1 << tp);
+        uint32_t tile0 = 0;
+        for (int i = 0; i < f; ++i)
+            if ((c0 >> i) & 1) tile0 |= 1u << qs[i];
+        const T *M = (cm != 0 && (tile0 & cm) == cm) ? op.m1 : op.m0;
+        // column c: out[c0] += M[0][tb] x_c, out[c1] += M[1][tb] x_c
+        m[2 * (size_t(c0) * D + c)] = (double)M[2 * tb];
+        m[2 * (size_t(c0) * D + c) + 1] = (double)M[2 * tb + 1];
+        m[2 * (size_t(c1) * D + c)] = (double)M[4 + 2 * tb];
+        m[2 * (size_t(c1) * D + c) + 1] = (double)M[4 + 2 * tb + 1];
+    }
+}
+
+// a <- b a (complex D x D, row-major re/im)
+void cmatmul_left(const std::vector<double> &b, std::vector<double> &a, int D) {
+    std::vector<double> r(size_t(2) * D * D, 0.0);
+    for (int i = 0; i < D; ++i)
+        for (int l = 0; l < D; ++l) {
+            const double br = b[2 * (size_t(i) * D + l)], bi = b[2 * (size_t(i) * D + l) + 1];
+            if (br == 0.0 && bi == 0.0) continue;
+            for (int j = 0; j < D; ++j) {
+                const double ar = a[2 * (size_t(l) * D + j)], ai = a[2 * (size_t(l) * D + j) + 1];
+                r[2 * (size_t(i) * D + j)] += br * ar - bi * ai;
+                r[2 * (size_t(i) * D + j) + 1] += br * ai + bi * ar;
+            }
+        }
+    a.swap(r);
+}
+
+// Pass fusion (complex64): the pass's ops are clustered over their dependency DAG (two ops depend
+// when their target/control bits intersect) into blocks of at most cfg.width tile bits, each block
+// grown by the ready op that adds the fewest new bits (earliest on ties); a block whose MUX2 cost
+// exceeds the cost of one dense op becomes one dense unit (product of its ops in FP64), the others
+// keep their ops.  Only ops on disjoint bits change order, so the result equals the pass's product
+// of ops up to rounding.
+template <typename T>
+void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vector<DenseUnit> &dus,
+               const FuseCfg &cfg) {
+    const int n = (int)pops.size();
+    std::vector<uint32_t> qm(n);
+    std::vector<char> fusable(n);
+    for (int i = 0; i < n; ++i) {
+        if (dense_of[i] >= 0) {
+            const DenseUnit &du = dus[dense_of[i]];
+            uint32_t m = 0;
+            for (int q = 0; q < du.f; ++q) m |= 1u << du.q[q];
+            qm[i] = m;
+            fusable[i] = 0;
+        } else {
+            qm[i] = (1u << pops[i].t) | (uint32_t)pops[i].cmask;
+            fusable[i] = __builtin_popcount(qm[i]) <= cfg.width;
+        }
+    }
+    std::vector<int> npred(n, 0);
+    std::vector<std::vector<int>> succ(n);
+    int last[32];
+    for (int b = 0; b < 32; ++b) last[b] = -1;
+    for (int i = 0; i < n; ++i) {
+        std::vector<int> deps;
+        for (int b = 0; b < 32; ++b)
+            if ((qm[i] >> b) & 1u) {
+                if (last[b] >= 0 && std::find(deps.begin(), deps.end(), last[b]) == deps.end()) deps.push_back(last[b]);
+                last[b] = i;
+            }
+        npred[i] = (int)deps.size();
+        for (int d : deps) succ[d].push_back(i);
+    }
+    std::set<int> ready;
+    for (int i = 0; i < n; ++i)
+        if (npred[i] == 0) ready.insert(i);
+    std::vector<OpDev<T>> out;
+    std::vector<int> out_dense;
+    while (!ready.empty()) {
+        uint32_t S = 0;
+        std::vector<int> mem;
+        bool single = false;
+        while (true) {
+            int best = -1, bestadd = 99;
+            for (int i : ready) {
+                if (!fusable[i]) continue;
+                const int add = __builtin_popcount(qm[i] & ~S);
+                if (__builtin_popcount(S) + add > cfg.width) continue;
+                if (add < bestadd) {
+                    best = i;
+                    bestadd = add;
+                }
+            }
+            if (best < 0) {
+                if (!mem.empty()) break;
+                best = *ready.begin();   // a non-fusable op (wide controls or a dense record) alone
+                single = true;
+            }
+            ready.erase(best);
+            mem.push_back(best);
+            S |= qm[best];
+            for (int s2 : succ[best])
+                if (--npred[s2] == 0) ready.insert(s2);
+            if (single) break;
+        }
+        double mux = 0;
+        if (!single)
+            for (int i : mem) mux += kOpCost[pops[i].kind];
+        const int nq = __builtin_popcount(S);
+        const double dcost = std::max(4, nq) == 5 ? cfg.cost5 : cfg.cost4;
+        if (!single && mem.size() >= 2 && mux > dcost) {
+            std::vector<int> qs;
+            for (int b = 0; b < 32; ++b)
+                if ((S >> b) & 1u) qs.push_back(b);
+            const int D = 1 << nq;
+            DenseUnit du;
+            du.f = nq;
+            for (int q = 0; q < nq; ++q) du.q[q] = qs[q];
+            du.u.assign(size_t(2) * D * D, 0.0);
+            for (int d = 0; d < D; ++d) du.u[2 * (size_t(d) * D + d)] = 1.0;
+            std::vector<double> m;
+            for (int i : mem) {
+                mux_matrix(pops[i], qs, m);
+                cmatmul_left(m, du.u, D);
+            }
+            OpDev<T> od{};
+            od.kind = -1;
+            od.t = qs[0];
+            out.push_back(od);
+            out_dense.push_back((int)dus.size());
+            dus.push_back(std::move(du));
+        } else {
+            for (int i : mem) {
+                out.push_back(pops[i]);
+                out_dense.push_back(dense_of[i]);
+            }
+        }
+    }
+    pops.swap(out);
+    dense_of.swap(out_dense);
+}
+
 template <typename T>
 int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     const int k = seg.k;
@@ -275,6 +442,21 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     // exact); SVX_DENSE_MATH=f32|f16 forces one of them
     const char *denv = std::getenv("SVX_DENSE_MATH");
     const std::string dense_math = denv ? denv : "auto";
+    // pass fusion of MUX2 ops into dense units (complex64, single-buffer kernel): "auto" fuses the
+    // passes too heavy for the double-buffered kernel, "on" every pass, "off" none;
+    // SVX_FUSE_COST="c4,c5[,width]" overrides the dense-op costs and the block width
+    const char *fenv = std::getenv("SVX_FUSE");
+    const std::string fuse_mode = fenv ? fenv : "auto";
+    FuseCfg fcfg;
+    if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
+        double c4 = fcfg.cost4, c5 = fcfg.cost5;
+        int wd = fcfg.width;
+        if (std::sscanf(cenv, "%lf,%lf,%d", &c4, &c5, &wd) >= 2) {
+            fcfg.cost4 = c4;
+            fcfg.cost5 = c5;
+            fcfg.width = std::min(5, std::max(1, wd));
+        }
+    }
     const bool dma_ok = dtype == SVX_C64 && (k == 12 || k == 13) && nbits >= k + 9;
     if ((pick == "dma" || pick == "pers" || pick == "dma5") && !dma_ok && !small)
         return fail(SVX_E_ARG, "SVX_PASS_KERNEL=" + pick + " needs complex64, k in {12, 13} and >= 2^(k+9) amplitudes");
@@ -298,7 +480,8 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     for (const RawPass &rp : seg.passes) {
         // classify
         std::vector<OpDev<T>> pops;
-        std::vector<const RawOp *> dense_of;   // parallel to pops: the dense record, or null
+        std::vector<DenseUnit> dus;
+        std::vector<int> dense_of;   // parallel to pops: index into dus, or -1 for a MUX2 op
         double cost = 0;
         bool has_dense = false;
         for (const RawOp &ro : rp.ops) {
@@ -309,12 +492,17 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 od.kind = -1;
                 od.t = ro.dq[0];
                 pops.push_back(od);
-                dense_of.push_back(&ro);
+                DenseUnit du;
+                du.f = ro.dense_f;
+                for (int q = 0; q < du.f; ++q) du.q[q] = ro.dq[q];
+                du.u.assign(ro.dm.begin(), ro.dm.end());
+                dense_of.push_back((int)dus.size());
+                dus.push_back(std::move(du));
                 has_dense = true;
                 cost += 8.0 * (ro.dense_f <= 4 ? 16 : 32) / 4.0;
                 continue;
             }
-            dense_of.push_back(nullptr);
+            dense_of.push_back(-1);
             OpDev<T> od{};
             for (int e = 0; e < 8; ++e) {
                 od.m0[e] = T(ro.f[e]);
@@ -342,9 +530,13 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             } else {
                 od.kind = K_MUX;
             }
-            static const double kcost[] = {8, 4, 1, 8, 4, 2, 2, 8};
-            cost += kcost[od.kind];
+            cost += kOpCost[od.kind];
             pops.push_back(od);
+        }
+        if (dtype == SVX_C64 && !small && fuse_mode != "off" && (pick == "auto" || pick == "v1") &&
+            (fuse_mode == "on" || pick == "v1" || !dma_ok || cost > kDmaMaxCost)) {
+            fuse_pass(pops, dense_of, dus, fcfg);
+            for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
         int mode;   // 0 small, 1 single-buffer, 2 DMA, 3 persistent
         if (small) mode = 0;
@@ -405,14 +597,14 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         if (mode == 0) {
             for (size_t i = 0; i < pops.size(); ++i) {
                 OpDev<T> o = pops[i];
-                if (dense_of[i]) {   // raw matrix for the small-state kernel's direct product
-                    const RawOp &ro = *dense_of[i];
+                if (dense_of[i] >= 0) {   // raw matrix for the small-state kernel's direct product
+                    const DenseUnit &du = dus[dense_of[i]];
                     GroupDev gd{};
                     gd.type = 2;
-                    gd.f = ro.dense_f;
-                    for (int q = 0; q < ro.dense_f; ++q) gd.dq[q] = gd.dqs[q] = ro.dq[q];
+                    gd.f = du.f;
+                    for (int q = 0; q < du.f; ++q) gd.dq[q] = gd.dqs[q] = du.q[q];
                     gd.w_off = (int64_t)wmat.size();
-                    wmat.insert(wmat.end(), ro.dm.begin(), ro.dm.end());
+                    for (double v : du.u) wmat.push_back((float)v);
                     o.kind = K_DENSE;
                     o.cmask = (int32_t)groups.size();
                     groups.push_back(gd);
@@ -426,12 +618,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         // groups: maximal runs of ops whose targets fit R register bits; a dense op is a group of its own
         size_t i = 0;
         while (i < pops.size()) {
-            if (dense_of[i]) {
-                const RawOp &ro = *dense_of[i];
-                const int f0 = ro.dense_f, F = std::max(4, f0);
+            if (dense_of[i] >= 0) {
+                const DenseUnit &du = dus[dense_of[i]];
+                const int f0 = du.f, F = std::max(4, f0);
                 GroupDev gd{};
                 gd.f = F;
-                std::vector<int> qs(ro.dq, ro.dq + f0);
+                std::vector<int> qs(du.q, du.q + f0);
                 for (int b = 0; (int)qs.size() < F && b < K; ++b)
                     if (std::find(qs.begin(), qs.end(), b) == qs.end()) qs.push_back(b);
                 if ((int)qs.size() < F) return fail(SVX_E_CAPACITY, "tile too small for a dense op");
@@ -439,12 +631,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 std::sort(gd.dqs, gd.dqs + F);
                 // padded matrix: the extra qubits are the high index bits (identity on them)
                 const int D = 1 << F, d0 = 1 << f0;
-                auto U = [&](int r, int c, int part) -> float {
-                    if ((r >> f0) != (c >> f0)) return 0.f;
-                    return ro.dm[2 * ((r & (d0 - 1)) * d0 + (c & (d0 - 1))) + part];
+                auto U = [&](int r, int c, int part) -> double {
+                    if ((r >> f0) != (c >> f0)) return 0.0;
+                    return du.u[2 * ((r & (d0 - 1)) * d0 + (c & (d0 - 1))) + part];
                 };
                 // real form W = [[Ur, -Ui], [Ui, Ur]] (2D x 2D) acting on [Re x; Im x]
-                auto W = [&](int row, int col) -> float {
+                auto W = [&](int row, int col) -> double {
                     const int r = row % D, c = col % D;
                     const bool rim = row >= D, cim = col >= D;
                     if (rim == cim) return U(r, c, 0);
@@ -452,7 +644,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 };
                 // entries in {0, +-1}: the FP32 products are exact (permutations stay bit-exact)
                 bool exact = true;
-                for (float v : ro.dm) exact = exact && (v == 0.f || v == 1.f || v == -1.f);
+                for (double v : du.u) exact = exact && (v == 0.0 || v == 1.0 || v == -1.0);
                 const int NRT = F == 5 ? 2 : 1;
                 while (wmat.size() % 4) wmat.push_back(0.f);   // 16-byte aligned operand blocks
                 gd.w_off = (int64_t)wmat.size();
@@ -461,7 +653,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     gd.type = 1;
                     for (int rt = 0; rt < NRT; ++rt)
                         for (int st = 0; st < D; ++st)
-                            for (int l = 0; l < 64; ++l) wmat.push_back(W(32 * rt + l % 32, 2 * st + l / 32));
+                            for (int l = 0; l < 64; ++l) wmat.push_back((float)W(32 * rt + l % 32, 2 * st + l / 32));
                 } else {
                     // split-FP16 A operands of v_mfma_f32_32x32x16_f16: 2^14 W = Wh + Wl; block
                     // ((rt NKS + ks) 2 + part) holds lane l, entry e = W[32 rt + l % 32][16 ks + 8 (l / 32) + e]
@@ -474,9 +666,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                             for (int part = 0; part < 2; ++part)
                                 for (int l = 0; l < 64; ++l)
                                     for (int e = 0; e < 8; ++e) {
-                                        const float ws = W(32 * rt + l % 32, 16 * ks + 8 * (l / 32) + e) * 16384.0f;
-                                        const uint16_t hh = f32_to_f16_bits(ws);
-                                        hb.push_back(part == 0 ? hh : f32_to_f16_bits(ws - f16_bits_to_f32(hh)));
+                                        const double ws = W(32 * rt + l % 32, 16 * ks + 8 * (l / 32) + e) * 16384.0;
+                                        const uint16_t hh = f32_to_f16_bits((float)ws);
+                                        hb.push_back(part == 0 ? hh : f32_to_f16_bits((float)(ws - f16_bits_to_f32(hh))));
                                     }
                     const size_t o = wmat.size();
                     wmat.resize(o + hb.size() / 2);
@@ -489,7 +681,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             std::vector<int> tg;
             size_t j = i;
             for (; j < pops.size(); ++j) {
-                if (dense_of[j]) break;
+                if (dense_of[j] >= 0) break;
                 const int t = pops[j].t;
                 if (std::find(tg.begin(), tg.end(), t) == tg.end()) {
                     if ((int)tg.size() == R) break;
@@ -562,7 +754,10 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     // the cache key includes the environment choices that change the descriptors
     const char *kenv = std::getenv("SVX_PASS_KERNEL");
     const char *denv = std::getenv("SVX_DENSE_MATH");
-    const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto");
+    const char *fenv = std::getenv("SVX_FUSE");
+    const char *cenv = std::getenv("SVX_FUSE_COST");
+    const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
+                             (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);
