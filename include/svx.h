@@ -76,6 +76,11 @@ int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches);
  * ..., "none" for a pass without effective ops), for profiling reports.  No reference counterpart. */
 int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
 
+/* Work of pass `pass` of a prepared segment after classification and pass fusion:
+ * out[0] MUX2 register groups, out[1] MUX2 ops, out[2] dense units on 4 qubits, out[3] dense units on
+ * 5 qubits, out[4] dense units on the exact FP32 matrix-core path.  No reference counterpart. */
+int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out);
+
 /* Qubit permutation perm[new_bit] = old_bit of a 2^n state, out of place (dst != src), bit-exact.
  * Replaces the reference's remap paths: model.remap (host/sv/model.py:74-78), the host transpose
  * of local qubits (host/sv/runtime.py:261-268) and the sv_swap bank exchange

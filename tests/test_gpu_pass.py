@@ -240,7 +240,7 @@ def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m, monkeypatch):
 
 @pytest.mark.parametrize("fuse,cost", [("off", None), ("auto", None), ("on", None), ("on", "0,0,5"),
                                        ("on", "0,0,3"), ("on", "0,0,1")])
-@pytest.mark.parametrize("n,k,max_m", [(12, 10, 4), (16, 12, 6), (20, 13, 7)])
+@pytest.mark.parametrize("n,k,max_m", [(12, 10, 4), (16, 12, 6), (20, 13, 7), (21, 12, 4), (22, 13, 7)])
 def test_pass_fusion_matches_oracle(gpu, n, k, max_m, fuse, cost, monkeypatch):
     """Pass fusion (MUX2 ops of a pass clustered into dense units on the matrix cores): every
     setting, from off to fusing every block of 1, 3 or 5 tile bits, matches the complex128 golden
@@ -256,13 +256,15 @@ def test_pass_fusion_matches_oracle(gpu, n, k, max_m, fuse, cost, monkeypatch):
     _check_c64(ref, got, (n, k, max_m, fuse, cost))
 
 
-def test_pass_fusion_of_permutations_is_exact(gpu, monkeypatch):
+@pytest.mark.parametrize("n,k", [(16, 12), (22, 13)])
+def test_pass_fusion_of_permutations_is_exact(gpu, n, k, monkeypatch):
     """Blocks of exact X / controlled-X ops fuse into 0/1 matrices, which run on the exact FP32
-    matrix-core path: the result equals the c64 golden model bit for bit."""
+    matrix-core path (single-buffer and double-buffered kernels): the result equals the c64 golden
+    model bit for bit."""
     monkeypatch.setenv("SVX_FUSE", "on")
     monkeypatch.setenv("SVX_FUSE_COST", "0,0,5")
     rng = np.random.default_rng(5)
-    seg = random_segment(rng, 16, 12, n_passes=3, max_ops=40, kinds=[("x", "x"), ("i", "x")], max_m=6)
-    s0 = _rand_state(rng, 16, np.complex64)
+    seg = random_segment(rng, n, k, n_passes=3, max_ops=40, kinds=[("x", "x"), ("i", "x")], max_m=6)
+    s0 = _rand_state(rng, n, np.complex64)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex64)
     assert np.array_equal(run_hip(s0, seg), ref)

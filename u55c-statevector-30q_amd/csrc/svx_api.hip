@@ -230,6 +230,7 @@ struct LaunchInfo {
     int64_t pass_index;   // index into the PassDev array
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
+    int32_t stats[5];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32)
 };
 
 struct Prepared {
@@ -442,9 +443,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     // exact); SVX_DENSE_MATH=f32|f16 forces one of them
     const char *denv = std::getenv("SVX_DENSE_MATH");
     const std::string dense_math = denv ? denv : "auto";
-    // pass fusion of MUX2 ops into dense units (complex64, single-buffer kernel): "auto" fuses the
-    // passes too heavy for the double-buffered kernel, "on" every pass, "off" none;
-    // SVX_FUSE_COST="c4,c5[,width]" overrides the dense-op costs and the block width
+    // pass fusion of MUX2 ops into dense units (complex64): "auto"/"on" fuse every pass where a
+    // block's MUX2 cost exceeds a dense op's, "off" none; SVX_FUSE_COST="c4,c5[,width]" overrides
+    // the dense-op costs and the block width
     const char *fenv = std::getenv("SVX_FUSE");
     const std::string fuse_mode = fenv ? fenv : "auto";
     FuseCfg fcfg;
@@ -533,18 +534,18 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             cost += kOpCost[od.kind];
             pops.push_back(od);
         }
-        if (dtype == SVX_C64 && !small && fuse_mode != "off" && (pick == "auto" || pick == "v1") &&
-            (fuse_mode == "on" || pick == "v1" || !dma_ok || cost > kDmaMaxCost)) {
+        if (dtype == SVX_C64 && !small && fuse_mode != "off" && (pick == "auto" || pick == "v1")) {
             fuse_pass(pops, dense_of, dus, fcfg);
             for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
         int mode;   // 0 small, 1 single-buffer, 2 DMA, 3 persistent
         if (small) mode = 0;
-        else if (pick == "v1" || !dma_ok || has_dense) mode = 1;
+        else if (pick == "v1" || !dma_ok) mode = 1;
+        else if (has_dense) mode = pick == "dma5" ? 2 : 1;   // dense groups: the single-buffer kernel
         else if (pick == "pers") mode = 3;
         else if (pick == "dma" || pick == "dma5") mode = 2;
         else mode = cost <= kDmaMaxCost ? 2 : 1;
-        const bool dma5 = mode == 2 && pick == "dma5" && k == 13;
+        const bool dma5 = mode == 2 && k == 13 && (pick == "dma5" || has_dense);
         const int K = mode == 0 ? nbits : (mode == 1 ? Kv1 : k);
         const int R = mode == 2 ? (dma5 ? 5 : 4) : (mode == 3 ? (k == 12 ? 4 : 5) : kR);
         PassDev pd{};
@@ -608,6 +609,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     o.kind = K_DENSE;
                     o.cmask = (int32_t)groups.size();
                     groups.push_back(gd);
+                    launches.back().stats[du.f == 5 ? 3 : 2] += 1;
+                } else {
+                    launches.back().stats[1] += 1;
                 }
                 ops.push_back(o);
             }
@@ -724,6 +728,17 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             i = j;
         }
         pd.ngroups = (int)groups.size() - pd.group0;
+        LaunchInfo &lr = launches.back();
+        for (int g = pd.group0; g < (int)groups.size(); ++g) {
+            const GroupDev &G = groups[g];
+            if (G.type == 0) {
+                lr.stats[0] += 1;
+                lr.stats[1] += G.nops;
+            } else {
+                lr.stats[G.f == 5 ? 3 : 2] += 1;
+                lr.stats[4] += G.type == 1;
+            }
+        }
         passes.push_back(pd);
     }
     // layout: passes, groups, ops (256-byte aligned sections)
@@ -806,7 +821,9 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
                                  reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
         else if (li.dma)
             e = launch_pass_dma(li.K + li.dma_variant, state, pd + li.pass_index, gd,
-                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
+                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
+                                li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr,
+                                li.ntiles, li.grid, li.lds, s);
         else if (pr->dtype == SVX_C64)
             e = launch_pass_c64(li.K, li.small, state, pd + li.pass_index, gd,
                                 reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
@@ -956,9 +973,19 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     if (li.empty) k = "none";
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
     else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
-    else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5>" : ",4>");
+    else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5" : ",4") + (li.dense ? ",dense>" : ">");
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());
+    return 0;
+}
+
+int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out) {
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
+        return fail(SVX_E_ARG, "unknown prepared-segment handle");
+    Prepared *pr = g_prepared[handle - 1].get();
+    if (pass < 0 || pass >= (int32_t)pr->launches.size() || !out) return fail(SVX_E_ARG, "bad pass index");
+    for (int i = 0; i < 5; ++i) out[i] = pr->launches[pass].stats[i];
     return 0;
 }
 

@@ -83,7 +83,7 @@ size_t pass_pers_lds_bytes(int K, int m);
 hipError_t launch_pass_pers(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
                             uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
-                           uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
+                           const float *wd, uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
 size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits);
 hipError_t launch_permute(int dtype, void *dst, const void *src, const PermDev *pd, uint32_t grid,
                           size_t lds, hipStream_t s);

@@ -305,49 +305,79 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
 }
 
 // ------------------------------------------------------------------------------------------ dense op
-// Fused form: a dense 2^F x 2^F complex op on workgroup-tile bits dq[0..F-1], applied to the LDS tile
-// with FP32 matrix cores.  Real form Y = W X with W = [[Ur, -Ui], [Ui, Ur]] (2^(F+1) square) and the
-// columns of X the 2^(K-F) groups of 2^F amplitudes (rows: Re x_0.., Im x_0..).  One wavefront
-// handles 32 columns per v_mfma_f32_32x32x2f32 chain: lane l owns column l % 32, half h = l / 32
-// feeds K-rows 2s + h; the accumulator lane layout of the 32x32 output is row i = 8 (v / 4) + 4 h +
-// v % 4 for accumulator v.  W arrives pre-arranged on the host in A-operand order:
-// wd[(rt * 2^F + s) * 64 + l] = W[32 rt + l % 32][2 s + l / 32].
+// Dense 2^F x 2^F complex op on workgroup-tile bits dq[0..F-1] (fused-form records and pass fusion),
+// applied to the LDS tile on the matrix cores.  Real form Y = W X with W = [[Ur, -Ui], [Ui, Ur]]
+// (2^(F+1) square) and the columns of X the 2^(K-F) groups of 2^F amplitudes (rows: Re x_0..,
+// Im x_0..).  A wavefront handles 32 columns at a time (lane l owns column l % 32); the accumulator
+// layout of a 32x32 output block is row i = 8 (v / 4) + 4 (l / 32) + v % 4 for accumulator v.
+//
+// The LDS tile layout is a template choice: padded (byte address 8 lpad(e), additive over disjoint
+// bits: the single-buffer kernel) or XOR-swizzled (swz(e), XOR-linear: the double-buffered kernel).
 typedef float f16v __attribute__((ext_vector_type(16)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
-template <int K, int F>
-__device__ __forceinline__ void dense_apply(C2<float> *tile, const GroupDev &G, const float *__restrict__ wd,
-                                            uint32_t tid) {
+__device__ __forceinline__ uint32_t swz(uint32_t e) {
+    return (((e >> 1) ^ ((e >> 5) & 15u)) << 4) | ((e & 1u) << 3);
+}
+
+template <bool SWZ>
+struct TileAddr {
+    __device__ __forceinline__ static uint32_t of(uint32_t e) { return SWZ ? swz(e) : 8u * lpad(e); }
+    __device__ __forceinline__ static uint32_t comb(uint32_t a, uint32_t b) { return SWZ ? (a ^ b) : (a + b); }
+};
+
+// byte offsets of the 2^F components of one column, folded from the per-bit offsets
+template <int F, bool SWZ>
+struct ColOff {
+    uint32_t lq[F];
+    __device__ __forceinline__ void init(const GroupDev &G) {
+#pragma unroll
+        for (int i = 0; i < F; ++i) lq[i] = TileAddr<SWZ>::of(1u << G.dq[i]);
+    }
+    __device__ __forceinline__ uint32_t operator()(uint32_t c) const {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+            if ((c >> i) & 1u) o = TileAddr<SWZ>::comb(o, lq[i]);
+        return o;
+    }
+};
+
+__device__ __forceinline__ C2<float> ldc(const unsigned char *buf, uint32_t a) {
+    return *reinterpret_cast<const C2<float> *>(buf + a);
+}
+__device__ __forceinline__ void stc(unsigned char *buf, uint32_t a, C2<float> v) {
+    *reinterpret_cast<C2<float> *>(buf + a) = v;
+}
+
+// FP32 matrix cores (exact for 0 / +-1 matrices): v_mfma_f32_32x32x2f32, half h = l / 32 feeds K-rows
+// 2s + h; W arrives in A-operand order wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32].
+// NWAVE wavefronts (wt = 0 .. 64 NWAVE - 1) share the columns.
+template <int K, int F, int NWAVE, bool SWZ>
+__device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                            uint32_t wt) {
     constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
     constexpr int NS = 1 << F;                     // K-steps of 2 real rows
     constexpr int NX = NS / 2;                     // complex inputs per lane
-    constexpr uint32_t NCOL = 1u << (K - F);
-    constexpr uint32_t NB = NCOL / 32;             // 32-column batches per tile
-    const uint32_t lane = tid & 63u, wave = tid >> 6, h = lane >> 5, j = lane & 31u;
+    constexpr uint32_t NB = (1u << (K - F)) / 32;  // 32-column batches per tile
+    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
     const float *w = wd + G.w_off;
     float wa[NRT][NS];
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
         for (int s = 0; s < NS; ++s) wa[rt][s] = w[(rt * NS + s) * 64 + lane];
-    uint32_t lq[F];
-#pragma unroll
-    for (int i = 0; i < F; ++i) lq[i] = lpad(1u << G.dq[i]);
-    // padded LDS offset of component c (pad is linear over disjoint bits)
-    auto coff = [&](uint32_t c) {
-        uint32_t o = 0;
-#pragma unroll
-        for (int i = 0; i < F; ++i)
-            if ((c >> i) & 1u) o += lq[i];
-        return o;
-    };
-    for (uint32_t b = wave; b < NB; b += kNT / 64) {
+    ColOff<F, SWZ> coff;
+    coff.init(G);
+    const uint32_t hoff = h ? coff.lq[0] : 0u;
+    for (uint32_t b = wave; b < NB; b += NWAVE) {
         uint32_t base = b * 32u + j;
 #pragma unroll
         for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
-        const uint32_t pb = lpad(base);
+        const uint32_t pb = TileAddr<SWZ>::of(base);
         C2<float> x[NX];
 #pragma unroll
-        for (int s = 0; s < NX; ++s) x[s] = tile[pb + coff(2u * s) + (h ? lq[0] : 0u)];
+        for (int s = 0; s < NX; ++s) x[s] = ldc(buf, TileAddr<SWZ>::comb(pb, TileAddr<SWZ>::comb(coff(2u * s), hoff)));
         f16v acc[NRT];
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt) {
@@ -358,41 +388,40 @@ __device__ __forceinline__ void dense_apply(C2<float> *tile, const GroupDev &G, 
                 acc[rt] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[rt][s], bv, acc[rt], 0, 0, 0);
             }
         }
-        // lane half h holds components c = 8 a + 4 h + e
         if (F == 4) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                tile[pb + coff(c)] = C2<float>{acc[0][v], acc[0][v + 8]};
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[0][v + 8]});
             }
         } else {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                tile[pb + coff(c)] = C2<float>{acc[0][v], acc[NRT - 1][v]};
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[NRT - 1][v]});
             }
         }
     }
 }
 
-// Dense op on split-FP16 matrix cores: W (pre-scaled by 2^14 and split W = Wh + Wl on the host)
-// times a column block x scaled by a per-column power of two s (max |x s| < 2^15) and split
-// x s = xh + xl in FP16 (the split is exact in FP32); y = (Wh xh + Wh xl + Wl xh) / (2^14 s) with
-// FP32 accumulation.  The dropped term and the two-piece representations bound the relative error
-// per product by about 2^-21.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B) l % 32
-// and K entries 8 (l / 32) + 0..7 of each 16-deep K step.
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+// Split-FP16 matrix cores: W (pre-scaled by 2^14 and split W = Wh + Wl on the host) times a column
+// block x scaled by a per-column power of two s (max |x s| < 2^15) and split x s = xh + xl in FP16;
+// y = (Wh xh + Wh xl + Wl xh) / (2^14 s) with FP32 accumulation.  The dropped product and the
+// two-piece representations leave a relative error of about 2^-21 of the column's largest entry
+// per product.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B)
+// l % 32 and K entries 8 (l / 32) + 0..7 of each 16-deep K step.
+// v rounded to 11 significant bits, to nearest with ties away from zero (a carry moves into the exponent)
+__device__ __forceinline__ float round11(float v) { return __uint_as_float((__float_as_uint(v) + 0x1000u) & 0xffffe000u); }
 
-template <int K, int F>
-__device__ __forceinline__ void dense_apply_h(C2<float> *tile, const GroupDev &G, const float *__restrict__ wd,
-                                              uint32_t tid) {
+template <int K, int F, int NWAVE, bool SWZ>
+__device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                              uint32_t wt) {
     constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
     constexpr int D = 1 << F;
     constexpr int NG = D / 16;                     // groups of 8 components per lane half
     constexpr int NKS = 2 * NG;                    // 16-deep K steps (real parts, then imaginary parts)
-    constexpr uint32_t NCOL = 1u << (K - F);
-    constexpr uint32_t NB = NCOL / 32;
-    const uint32_t lane = tid & 63u, wave = tid >> 6, h = lane >> 5, j = lane & 31u;
+    constexpr uint32_t NB = (1u << (K - F)) / 32;
+    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
     const half8 *w8 = reinterpret_cast<const half8 *>(wd + G.w_off);
     half8 ah[NRT][NKS], al[NRT][NKS];
 #pragma unroll
@@ -402,45 +431,53 @@ __device__ __forceinline__ void dense_apply_h(C2<float> *tile, const GroupDev &G
             ah[rt][ks] = w8[((rt * NKS + ks) * 2 + 0) * 64 + lane];
             al[rt][ks] = w8[((rt * NKS + ks) * 2 + 1) * 64 + lane];
         }
-    uint32_t lq[F];
-#pragma unroll
-    for (int i = 0; i < F; ++i) lq[i] = lpad(1u << G.dq[i]);
-    auto coff = [&](uint32_t c) {
-        uint32_t o = 0;
-#pragma unroll
-        for (int i = 0; i < F; ++i)
-            if ((c >> i) & 1u) o += lq[i];
-        return o;
-    };
-    for (uint32_t b = wave; b < NB; b += kNT / 64) {
+    ColOff<F, SWZ> coff;
+    coff.init(G);
+    for (uint32_t b = wave; b < NB; b += NWAVE) {
         uint32_t base = b * 32u + j;
 #pragma unroll
         for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
-        const uint32_t pb = lpad(base);
+        const uint32_t pb = TileAddr<SWZ>::of(base);
         C2<float> x[NG][8];
         float mx = 0.f;
 #pragma unroll
         for (int g = 0; g < NG; ++g)
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                x[g][e] = tile[pb + coff(16u * g + 8u * h + e)];
+                x[g][e] = ldc(buf, TileAddr<SWZ>::comb(pb, coff(16u * g + 8u * h + e)));
                 mx = fmaxf(mx, fmaxf(fabsf(x[g][e].x), fabsf(x[g][e].y)));
             }
         mx = fmaxf(mx, __shfl_xor(mx, 32));
         int ex = 0;
         frexpf(mx, &ex);                           // mx < 2^ex
         const int sh = mx > 0.f ? 15 - ex : 0;
+        // x s = xh + xl: xh = x s rounded to 11 significant bits (nearest, ties away from zero, by
+        // an integer add and mask; an exact FP16 value down to 2^-14, i.e. 2^-28 of the column
+        // maximum), xl = x s - xh (exact in FP32) converted toward zero.  xl's sign does not follow
+        // x's, so the truncation does not shrink amplitudes on average and the norm does not drift
+        // over many dense ops.
         half8 bh[NKS], bl[NKS];
 #pragma unroll
         for (int g = 0; g < NG; ++g)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float xr = ldexpf(x[g][e].x, sh), xi = ldexpf(x[g][e].y, sh);
-                const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
-                bh[g][e] = hr;
-                bl[g][e] = (_Float16)(xr - (float)hr);
-                bh[NG + g][e] = hi;
-                bl[NG + g][e] = (_Float16)(xi - (float)hi);
+            for (int e = 0; e < 8; e += 2) {
+                float xs[2][2], xh[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    xs[0][u] = ldexpf(x[g][e + u].x, sh);
+                    xs[1][u] = ldexpf(x[g][e + u].y, sh);
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) xh[p][u] = round11(xs[p][u]);
+                }
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const auto hp = __builtin_amdgcn_cvt_pkrtz(xh[p][0], xh[p][1]);
+                    const auto lp = __builtin_amdgcn_cvt_pkrtz(xs[p][0] - xh[p][0], xs[p][1] - xh[p][1]);
+                    bh[p * NG + g][e] = hp[0];
+                    bh[p * NG + g][e + 1] = hp[1];
+                    bl[p * NG + g][e] = lp[0];
+                    bl[p * NG + g][e + 1] = lp[1];
+                }
             }
         f16v acc[NRT];
 #pragma unroll
@@ -460,15 +497,30 @@ __device__ __forceinline__ void dense_apply_h(C2<float> *tile, const GroupDev &G
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                tile[pb + coff(c)] = C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[0][v + 8], back)};
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)),
+                    C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[0][v + 8], back)});
             }
         } else {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                tile[pb + coff(c)] = C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[NRT - 1][v], back)};
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)),
+                    C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[NRT - 1][v], back)});
             }
         }
+    }
+}
+
+// One dense group (type 1: FP32 matrix cores, type 3: split FP16) by NWAVE wavefronts
+template <int K, int NWAVE, bool SWZ>
+__device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                            uint32_t wt) {
+    if (G.type == 1) {
+        if (G.f == 5) dense_apply<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
+        else dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
+    } else {
+        if (G.f == 5) dense_apply_h<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
+        else dense_apply_h<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
     }
 }
 
@@ -508,15 +560,8 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
     for (int g = 0; g < ng; ++g) {
         const GroupDev &G = gd[g0 + g];
         if constexpr (DN && sizeof(T) == 4) {
-            if (G.type == 1) {
-                if (G.f == 5) dense_apply<K, 5>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
-                else dense_apply<K, 4>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
-                __syncthreads();
-                continue;
-            }
-            if (G.type == 3) {
-                if (G.f == 5) dense_apply_h<K, 5>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
-                else dense_apply_h<K, 4>(reinterpret_cast<C2<float> *>(tile), G, wd, tid);
+            if (G.type != 0) {
+                dense_group<K, kNT / 64, false>(smem, G, wd, tid);
                 __syncthreads();
                 continue;
             }
@@ -566,20 +611,17 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
 // for the disjoint bit sets of a register group.
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-__device__ __forceinline__ uint32_t swz(uint32_t e) {
-    return (((e >> 1) ^ ((e >> 5) & 15u)) << 4) | ((e & 1u) << 3);
-}
-
 __device__ __forceinline__ void lds_barrier() {
     // LDS writes of this wave complete, then the workgroup barrier; global stores stay in flight
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 
-template <int K, int R, int NW>
+template <int K, int R, int NW, bool DN>
 __global__ __launch_bounds__(64 + NW) void k_pass_dma(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                                    const GroupDev *__restrict__ gd,
-                                                   const OpDev<float> *__restrict__ od, uint64_t ntiles) {
+                                                   const OpDev<float> *__restrict__ od, const float *__restrict__ wd,
+                                                   uint64_t ntiles) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr uint32_t NE = 1u << K, NS = NE / 2, BUF = NE * 8;
     constexpr uint32_t kNTD = 64 + NW;       // loader wave + NW worker threads
@@ -645,6 +687,13 @@ __global__ __launch_bounds__(64 + NW) void k_pass_dma(C2<float> *__restrict__ st
             unsigned char *buf = smem + cur * BUF;
             for (int g = 0; g < ng; ++g) {
                 const GroupDev &Gd = gd[g0 + g];
+                if constexpr (DN) {
+                    if (Gd.type != 0) {
+                        dense_group<K, NW / 64, true>(buf, Gd, wd, w);
+                        lds_barrier();
+                        continue;
+                    }
+                }
                 uint32_t boff[NR];
                 boff[0] = 0;
 #pragma unroll
@@ -792,12 +841,20 @@ size_t pass_dma_lds_bytes(int K, int m) {
 }
 
 hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
-                           uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s) {
+                           const float *wd, uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s) {
     C2<float> *p = reinterpret_cast<C2<float> *>(st);
+    if (wd) {   // pass with dense groups
+        switch (K) {
+        case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
+        case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (K) {
-    case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, ntiles); break;
-    case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, ntiles); break;
-    case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, ntiles); break;
+    case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, false>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
+    case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512, false>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, wd, ntiles); break;
+    case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256, false>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1259,7 +1316,9 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<float, 13, true>)) SVX_ATTR((k_pass<float, 14, true>))
     SVX_ATTR((k_pass<double, 10, false>)) SVX_ATTR((k_pass<double, 11, false>)) SVX_ATTR((k_pass<double, 12, false>))
     SVX_ATTR((k_pass<double, 13, false>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
-    SVX_ATTR((k_pass_dma<12, 4, 256>)) SVX_ATTR((k_pass_dma<13, 4, 512>)) SVX_ATTR((k_pass_dma<13, 5, 256>))
+    SVX_ATTR((k_pass_dma<12, 4, 256, false>)) SVX_ATTR((k_pass_dma<13, 4, 512, false>))
+    SVX_ATTR((k_pass_dma<13, 5, 256, false>)) SVX_ATTR((k_pass_dma<12, 4, 256, true>))
+    SVX_ATTR((k_pass_dma<13, 5, 256, true>))
     SVX_ATTR((k_pass_pers<12, 4, 256>)) SVX_ATTR((k_pass_pers<13, 5, 256>))
 #undef SVX_ATTR
     return e;

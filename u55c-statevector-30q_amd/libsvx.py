@@ -19,7 +19,7 @@ C64, C128 = 0, 1
 SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -1, -2, -3, -4, -5, -6
 
 SYMBOLS = ["svx_init", "svx_run_segment", "svx_engine", "svx_prepare", "svx_run_prepared",
-           "svx_prepared_info", "svx_prepared_kernel", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
+           "svx_prepared_info", "svx_prepared_kernel", "svx_prepared_pass_stats", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
            "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync", "svx_last_error",
            "svx_version"]
 
@@ -45,6 +45,7 @@ _SIGS = {
     "svx_run_prepared": ([_VP, _I64, _VP], _I32),
     "svx_prepared_info": ([_I64, _PI32, _PI32], _I32),
     "svx_prepared_kernel": ([_I64, _I32, ctypes.c_char_p, _I32], _I32),
+    "svx_prepared_pass_stats": ([_I64, _I32, _PI32, ], _I32),
     "svx_permute": ([_VP, _VP, _I32, _PI32, _I32, _VP], _I32),
     "svx_swap_words": ([_VP, _VP, _I64, _I64, _I64, _VP], _I32),
     "svx_zero_state": ([_VP, _I32, _I32, _VP], _I32),
@@ -172,6 +173,15 @@ def prepared_kernel(handle, pass_index):
     buf = ctypes.create_string_buffer(64)
     _check(lib.svx_prepared_kernel(handle, pass_index, buf, 64))
     return buf.value.decode()
+
+
+def prepared_pass_stats(handle, pass_index):
+    """Work of one pass after classification and fusion: MUX2 groups, MUX2 ops, dense units on 4
+    and 5 qubits, dense units on the exact FP32 path."""
+    lib = load()
+    out = (ctypes.c_int32 * 5)()
+    _check(lib.svx_prepared_pass_stats(handle, pass_index, out))
+    return dict(mux_groups=out[0], mux_ops=out[1], dense4=out[2], dense5=out[3], dense_exact=out[4])
 
 
 def permute(dst, src, perm, stream=None):
