@@ -1,16 +1,20 @@
-"""Benchmark: 30-qubit complex64 plain-MUX2 pass bandwidth on one MI355X.
+"""Benchmark: 30-qubit complex64 MUX2 pass bandwidth on one MI355X, fused vs plain time at 32 and
+34 qubits.
 
-Workload: the 30-qubit layer circuit (U3 on every qubit with angles U[0, 2pi), then CX on a seeded
-matching in which each of qubits 26..29 is paired with one of 0..25; depth 20, seed 1), compiled
-by this project's compiler for the MI355X tile (2^13 amplitudes, up to 7 gathered high qubits).
-A step applies every pass of the compiled circuit once to the 8 GiB state resident in HBM (the
-state is a valid input for the next step, so nothing is re-initialised inside the timed region).
+Workload (value): the 30-qubit layer circuit (U3 on every qubit with angles U[0, 2pi), then CX on a
+seeded matching in which each of qubits 26..29 is paired with one of 0..25; depth 20, seed 1),
+compiled to plain MUX2 records by this project's compiler for the MI355X tile (2^13 amplitudes, up
+to 7 gathered high qubits).  The library runs the records with its default pass fusion (each
+pass's ops clustered into dense units on the matrix cores).  A step applies every pass once to the
+8 GiB state resident in HBM (the state is a valid input for the next step, so nothing is
+re-initialised inside the timed region).
 
 value = passes x 2 x 2^30 x 8 B / time per step (the reference's sweep-bandwidth definition,
 analysis/make_tables.py:320-321).  roofline: the pass kernel's algorithmic bytes per launch over
 its mean duration from HIP events on the launch stream.  cpu_baseline: the numpy golden model
 (oracle/model.py, the reference's CPU path) on the 20-qubit workload, one pinned core, in a
-child process.
+child process.  fused_vs_plain: the 32- and 34-qubit depth-40 workloads, plain MUX2 (vector
+units only) against the autotuned form, with the complex128 plain run as judge at 32 qubits.
 """
 import argparse
 import json
@@ -122,6 +126,67 @@ def cp_ladder_section(libsvx, circuit, compiler, torch, np, n=30, seed=300):
             "error_growth_pass_1mF_maxabs": [[g[0], float(f"{g[1]:.3e}"), float(f"{g[2]:.3e}")] for g in growth]}
 
 
+def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
+    """Fused vs plain on the n-qubit fused-form workload (layer recipe, depth 40, seed 2),
+    complex64: the plain MUX2 records with every op on the vector units against the autotuned
+    form (tile width x {plain, library fusion, fused 4-/5-qubit records}, timed on a depth-4
+    calibration circuit).  With `judge`, the complex128 run of the plain records judges the
+    fused state (FP64 fidelity on the device)."""
+    qc = circuit.workload_fused(n)
+    free, _ = torch.cuda.mem_get_info(0)
+    need = (1 << n) * (8 + (16 if judge else 0))
+    if need > free * 0.95:
+        return {"skipped": f"needs {need / 2**30:.0f} GiB of device memory, {free / 2**30:.0f} GiB free"}
+    state = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
+    tuner = runtime.Autotuner()
+    t0 = time.perf_counter()
+    rec = tuner.tune(state, arch)
+    t_tune = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+
+    def timed(k, form, dense_math="auto"):
+        prog, fmap, st = runtime.compile_form(qc, k, form)
+        os.environ["SVX_DENSE_MATH"] = dense_math
+        with libsvx.fusion(runtime.form_fusion(form)):
+            hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
+        os.environ.pop("SVX_DENSE_MATH")
+        libsvx.zero_state(state)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for h in hs:
+            libsvx.run_prepared(state, h)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), st, fmap, prog
+
+    plain_ms, st_p, fmap_p, prog_p = timed(13, "plain")
+    f32_ms = None
+    if rec["form"] != "plain":
+        f32_ms, _, _, _ = timed(rec["k"], rec["form"], "f32")      # dense units on FP32 matrix cores
+    fused_ms, st_f, fmap_f, prog_f = timed(rec["k"], rec["form"])
+    out = {"workload": f"u3cx_{n}q_d40_s2: {n} qubits, complex64, depth 40, seed 2 ({st_p.n_ops} MUX2 ops)",
+           "plain_ms": round(plain_ms, 1), "plain_passes": st_p.n_passes,
+           "fused_ms": round(fused_ms, 1), "fused_passes": st_f.n_passes,
+           "fused_dense_records": getattr(st_f, "n_dense", 0),
+           "speedup": round(plain_ms / fused_ms, 3),
+           "fused_fp32_mfma_ms": None if f32_ms is None else round(f32_ms, 1),
+           "tuned": {"k": rec["k"], "form": rec["form"], "tune_s": round(t_tune, 1),
+                     "calibration_ms": rec["table"]}}
+    if judge:
+        assert fmap_p == fmap_f == list(range(n)), "the layer recipe compiles without relabelling"
+        s128 = torch.empty(1 << n, dtype=torch.complex128, device="cuda:0")
+        libsvx.zero_state(s128)
+        for _, w in prog_p:
+            libsvx.run_segment(s128, w)
+        out["fused_vs_c128_plain"] = libsvx.fidelity_f64(s128, state)
+        norm, _ = libsvx.reduce_f64(state)
+        out["fused_norm"] = norm
+        del s128
+    del state
+    torch.cuda.empty_cache()
+    return out
+
+
 def _run_cpu_child():
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"],
                        capture_output=True, text=True, timeout=600)
@@ -137,6 +202,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-judge", action="store_true")
+    ap.add_argument("--no-fused", action="store_true", help="skip the 32/34-qubit fused-vs-plain section")
     ap.add_argument("--cpu-baseline-child", action="store_true")
     a = ap.parse_args()
     if a.cpu_baseline_child:
@@ -150,7 +216,7 @@ def main():
     import torch
     import __graft_entry__ as ge
     ge.load_package()
-    from u55c_statevector_30q_amd import circuit, compiler, libsvx
+    from u55c_statevector_30q_amd import circuit, compiler, libsvx, runtime
 
     caps = libsvx.init(0)
     n = 30
@@ -215,9 +281,16 @@ def main():
         torch.cuda.empty_cache()
 
     ladder = None if a.no_judge else cp_ladder_section(libsvx, circuit, compiler, torch, np)
+    arch = caps.arch.decode().split(":")[0]
+    fused = None
+    if not a.no_fused:
+        del state
+        torch.cuda.empty_cache()
+        fused = {str(nq): fused_section(libsvx, circuit, runtime, torch, nq, arch, judge=(nq == 32 and not a.no_judge))
+                 for nq in (32, 34)}
     cpu = None if a.no_cpu else _run_cpu_child()
     out = {
-        "metric": "30q c64 pass GB/s (% of 8 TB/s)",
+        "metric": "30q c64 pass GB/s (% of 8 TB/s); fused vs plain time; served circuits/s+p99",
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -246,6 +319,7 @@ def main():
                          "n": len(per_pass_gbs)},
         "judge_vs_c128": judge,
         "cp_ladder_30q": ladder,
+        "fused_vs_plain": fused,
         "cpu_baseline": cpu,
         "device": caps.arch.decode(),
     }

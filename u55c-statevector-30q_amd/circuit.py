@@ -255,6 +255,11 @@ def workload_gpu30():
     return u3_cx_layers(30, 20, 1, force_high=(26, 27, 28, 29))
 
 
+def workload_fused(n):
+    """The fused-form workload: n qubits (32 or 34), depth 40, seed 2, same layer recipe."""
+    return u3_cx_layers(n, 40, 2)
+
+
 def cp_ladder(n):
     """Controlled-phase ladder: for j = 0..n-1, H(j) then CP(pi / 2^(k-j)) between control k and
     target j for every k > j (n H + n(n-1)/2 CP).  The final bit reversal is a separate

@@ -148,6 +148,28 @@ def engine(buf_u8, ins_off, stream=None):
     _check(lib.svx_engine(_dev(buf_u8), int(ins_off), _stream(stream)))
 
 
+class fusion:
+    """Context manager selecting the library's pass fusion for segments prepared inside it
+    (SVX_FUSE: "auto" clusters MUX2 ops into dense units, "off" runs every op as MUX2)."""
+
+    def __init__(self, mode):
+        if mode not in ("auto", "on", "off"):
+            raise ValueError("fusion mode must be auto, on or off")
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = os.environ.get("SVX_FUSE")
+        os.environ["SVX_FUSE"] = self.mode
+        return self
+
+    def __exit__(self, *exc):
+        if self.prev is None:
+            os.environ.pop("SVX_FUSE", None)
+        else:
+            os.environ["SVX_FUSE"] = self.prev
+        return False
+
+
 def prepare(words, dtype, state_bits):
     lib = load()
     w, wp = _i32(words)

@@ -11,12 +11,15 @@ to 7 gathered high qubits, DAG-aware pass packing); target="reference" keeps the
 plan (P slices of 2^L, k <= 12, m <= max_high) and runs its stream unchanged -- P slices are one
 device buffer whose top log2(P) index bits are the reference's bank qubits.
 """
+import json
+import os
 import time
 
 import numpy as np
 
+from . import circuit as circuit_mod
 from . import compiler, libsvx
-from .compiler import compile_circuit, compile_gpu
+from .compiler import compile_circuit, compile_fused, compile_gpu
 
 KMAX = 12          # host/sv/runtime.py:26-32
 LPREF = 17
@@ -52,12 +55,101 @@ def sample_probabilities(state, shots, fmap, n, seed=None, chunk_bits=12):
     return {int(v): int(c) for v, c in zip(vals, cnt)}
 
 
+FORMS = ("plain", "lib", "w4", "w5")
+TUNE_KS = (12, 13, 14)
+
+
+def compile_form(qc, k, form, max_high=None):
+    """Compile for one execution form: "plain" (MUX2 records, every op on the vector units),
+    "lib" (MUX2 records, the library clusters each pass's ops into dense units), "w4" / "w5"
+    (fused-form records: the compiler emits dense 4- / 5-qubit units)."""
+    if form in ("plain", "lib"):
+        return compile_gpu(qc, k=k, max_high=max_high)
+    if form in ("w4", "w5"):
+        return compile_fused(qc, k=k, width=int(form[1]), max_high=max_high)
+    raise ValueError(f"unknown form {form!r}")
+
+
+def form_fusion(form):
+    """Library pass-fusion mode for a form (plain records run unfused for "plain")."""
+    return "off" if form == "plain" else "auto"
+
+
+class Autotuner:
+    """Per (qubit count, dtype, device) tuning record: the tile width k and execution form with the
+    shortest sweep time on a calibration circuit of the layer recipe (depth `depth`, its own seed),
+    measured with HIP events on the resident state.  Records persist as JSON when `path` is set."""
+
+    def __init__(self, path=None, ks=TUNE_KS, forms=FORMS, depth=4, seed=7):
+        self.path = path
+        self.ks = tuple(ks)
+        self.forms = tuple(forms)
+        self.depth = depth
+        self.seed = seed
+        self.records = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.records = json.load(f)
+
+    @staticmethod
+    def key(n, dtype, arch):
+        return f"{arch}/{dtype}/{n}"
+
+    def lookup(self, n, dtype, arch):
+        return self.records.get(self.key(n, dtype, arch))
+
+    def tune(self, state, arch, max_high=None):
+        """Time every (k, form) candidate on `state` (a complex64 device tensor of 2^n amplitudes,
+        overwritten) and keep the fastest.  complex128 states are not tuned: they run plain."""
+        import torch
+        n = state.numel().bit_length() - 1
+        dt = "c64" if state.dtype == torch.complex64 else "c128"
+        if dt != "c64":
+            rec = dict(k=compiler.GPU_K, form="plain", ms=None, table={})
+            self.records[self.key(n, dt, arch)] = rec
+            return rec
+        qc = circuit_mod.u3_cx_layers(n, self.depth, self.seed)
+        stream = torch.cuda.current_stream()
+        table = {}
+        for k in self.ks:
+            if k > n:
+                continue
+            for form in self.forms:
+                prog, fmap, st = compile_form(qc, k, form, max_high)
+                if any(kind != "segment" for kind, _ in prog):
+                    continue
+                with libsvx.fusion(form_fusion(form)):
+                    hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
+                libsvx.zero_state(state)
+                for h in hs:                      # warm-up (first launch of each kernel)
+                    libsvx.run_prepared(state, h)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for h in hs:
+                    libsvx.run_prepared(state, h)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                table[f"k{k}/{form}"] = round(e0.elapsed_time(e1), 3)
+        best = min(table, key=table.get)
+        kk, form = best.split("/")
+        rec = dict(k=int(kk[1:]), form=form, ms=table[best], table=table,
+                   calibration=f"u3cx layers, depth {self.depth}, seed {self.seed}")
+        self.records[self.key(n, dt, arch)] = rec
+        if self.path:
+            with open(self.path, "w") as f:
+                json.dump(self.records, f, indent=1, sort_keys=True)
+        return rec
+
+
 class SVRuntime:
-    def __init__(self, device=0, n_engines=16, max_high=None, target="mi355x", dtype="c64"):
+    def __init__(self, device=0, n_engines=16, max_high=None, target="mi355x", dtype="c64", tuner=None):
         import torch
         self.torch = torch
         self.device = device
         self.caps = libsvx.init(device)
+        self.arch = self.caps.arch.decode().split(":")[0]
+        self.tuner = tuner
+        self.form = "lib"
         self.n_engines = n_engines
         self.target = target
         self.max_high = max_high
@@ -114,8 +206,18 @@ class SVRuntime:
             prog, fmap, stats = compile_circuit(qrun, L, kk, fuse_1q=fuse, tiling=tiling,
                                                 initial_map=initial_map, max_high=mh)
             return prog, fmap, stats, P, n_run, L, kk
-        kk = k or compiler.GPU_K
-        prog, fmap, stats = compile_gpu(qc, k=kk, max_high=self.max_high, fuse_1q=fuse)
+        kk, form = k or compiler.GPU_K, "lib"
+        dt = "c64" if self.dtype == self.torch.complex64 else "c128"
+        rec = self.tuner.lookup(n, dt, self.arch) if self.tuner is not None else None
+        if rec is not None and k is None:
+            kk, form = rec["k"], rec["form"]
+        if dt == "c128" or not fuse:
+            form = "plain" if form in ("w4", "w5") else form
+        self.form = form
+        if form in ("w4", "w5"):
+            prog, fmap, stats = compile_fused(qc, k=kk, width=int(form[1]), max_high=self.max_high, fuse_1q=fuse)
+        else:
+            prog, fmap, stats = compile_gpu(qc, k=kk, max_high=self.max_high, fuse_1q=fuse)
         return prog, fmap, stats, 1, n, n, min(kk, n)
 
     def execute(self, prog, n_run, P, stream=None, timing=None):
@@ -129,7 +231,8 @@ class SVRuntime:
             a = time.perf_counter()
             if kind == "segment":
                 started = True
-                libsvx.run_segment(self.st, payload, n_slices=P, stream=stream)
+                with libsvx.fusion(form_fusion(self.form) if self.target == "mi355x" else "auto"):
+                    libsvx.run_segment(self.st, payload, n_slices=P, stream=stream)
                 torch.cuda.synchronize(self.device)
                 T["T_kernel"] = T.get("T_kernel", 0.0) + time.perf_counter() - a
             else:
