@@ -135,12 +135,13 @@ def test_split_fp16_and_fp32_paths(gpu, scale, monkeypatch):
 def test_dense_records_double_buffered_kernel(gpu, n, k, monkeypatch):
     """The double-buffered kernel's dense instance (SVX_PASS_KERNEL=dma5: XOR-swizzled LDS tile,
     loader wave + worker waves) against the golden model and bit-identical to the single-buffer
-    kernel (same groups, same arithmetic)."""
+    kernel with per-column scaling (same groups, same arithmetic)."""
     rng = np.random.default_rng(5 * n + k)
     seg = dense_segment(rng, n, k, n_passes=2, max_m=4)
     s0 = _rand_state(rng, n)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
     monkeypatch.setenv("SVX_FUSE", "off")
+    monkeypatch.setenv("SVX_TILE_SCALE", "off")
     got = {}
     for pick in ("dma5", "v1"):
         monkeypatch.setenv("SVX_PASS_KERNEL", pick)

@@ -268,3 +268,60 @@ def test_pass_fusion_of_permutations_is_exact(gpu, n, k, monkeypatch):
     s0 = _rand_state(rng, n, np.complex64)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex64)
     assert np.array_equal(run_hip(s0, seg), ref)
+
+
+def _gain_segment(rng, L, k, n_passes, n_ops, gain, max_m=6):
+    """Passes of random (controlled) 2x2 ops multiplied by `gain` (non-unitary for gain != 1)."""
+    w = [MAGIC, L, n_passes, k, 0, 0, 0, 0]
+    for _ in range(n_passes):
+        m = int(rng.integers(0, min(max_m, k - 6) + 1))
+        hi = sorted(rng.choice(list(range(k - m, L)), size=m, replace=False).tolist()) if m else []
+        h = hi + [0] * (11 - len(hi))
+        w += [m, h[0], h[1], n_ops, h[2], h[3], 0, 0]
+        if m > 4:
+            w += [0x48] + h[4:11]
+        for _ in range(n_ops):
+            t = int(rng.integers(0, k))
+            c = int(rng.integers(0, k))
+            cm = 0 if c == t or rng.random() < 0.5 else 1 << c
+            w += _op_words(t, cm, gain * _rand_u(rng), gain * _rand_u(rng))
+    return (np.asarray(w, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+
+
+@pytest.mark.parametrize("gain,scale", [(1.0, 1.0), (1.0, 1e6), (1.0, 1e-25), (1.15, 1.0), (2.0, 1.0), (0.5, 1e3)])
+@pytest.mark.parametrize("tile_scale", ["auto", "off"])
+def test_pass_fusion_tile_scaling(gpu, gain, scale, tile_scale, monkeypatch):
+    """Split-FP16 dense groups on a pre-scaled tile (the pass kernel scales each tile by a power of
+    two from the tile maximum and the host's growth bound of the pass's matrices) and with
+    per-column scaling: unnormalised states far from unit norm and non-unitary ops that grow or
+    shrink the amplitudes within a pass match the complex128 golden model (judged on the
+    normalised states)."""
+    monkeypatch.setenv("SVX_FUSE", "on")
+    monkeypatch.setenv("SVX_TILE_SCALE", tile_scale)
+    rng = np.random.default_rng(int(gain * 100) + int(np.log10(scale)) + 50)
+    n, k = 18, 13
+    seg = _gain_segment(rng, n, k, n_passes=3, n_ops=24, gain=gain)
+    s0 = _rand_state(rng, n) * scale
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got = run_hip(s0.astype(np.complex64), seg)
+    assert np.all(np.isfinite(got))
+    _check_c64(ref, got, (gain, scale, tile_scale))
+    nr = np.linalg.norm(ref)
+    assert abs(np.linalg.norm(got.astype(np.complex128)) / nr - 1) < 1e-5
+
+
+@pytest.mark.parametrize("pick", ["dma", "dma5"])
+@pytest.mark.parametrize("n,k", [(21, 12), (22, 13)])
+def test_pass_fusion_double_buffered_kernels(gpu, n, k, pick, monkeypatch):
+    """Pass fusion on the double-buffered kernel's dense instances (8 worker waves with 4-bit
+    register groups, or 4 worker waves with 5-bit groups) against the complex128 golden model."""
+    if pick == "dma5" and k != 13:
+        pytest.skip("the 5-bit-group double-buffered kernel exists for 2^13-amplitude tiles")
+    monkeypatch.setenv("SVX_PASS_KERNEL", pick)
+    monkeypatch.setenv("SVX_FUSE", "on")
+    rng = np.random.default_rng(7 * n + k)
+    seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.12)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got = run_hip(s0.astype(np.complex64), seg)
+    _check_c64(ref, got, (n, k, pick))

@@ -432,11 +432,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         return fail(SVX_E_CAPACITY, "tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS tile limit 2^" +
                                         std::to_string(maxK));
     const bool small = nbits <= kSmallMaxBits;
-    // kernel choice per pass.  The double-buffered DMA kernel (k_pass_dma) needs complex64, one QSV2
-    // tile per LDS buffer (K = k in {12, 13}) and enough tiles to keep every CU busy; it overlaps
-    // HBM traffic with the gate math but its 4-bit register groups cost more per op, so "auto" uses
-    // it for passes whose estimated VALU cost is small and the single-buffer kernel (k_pass, 5-bit
-    // groups) otherwise.  SVX_PASS_KERNEL=v1|dma|dma5|pers|auto forces a choice.
+    // kernel choice per pass.  "auto" runs every pass on the single-buffer kernel (k_pass: two
+    // workgroups per CU, non-temporal 16-byte HBM access, 5-bit register groups, dense groups on the
+    // matrix cores), which measured fastest for light and heavy passes alike.  The double-buffered
+    // DMA kernel (k_pass_dma: loader wave + worker waves; complex64, K = k in {12, 13}, enough tiles
+    // to keep every CU busy) and the persistent variant stay selectable: SVX_PASS_KERNEL=v1|dma|
+    // dma5|pers|auto.
     const char *env = std::getenv("SVX_PASS_KERNEL");
     const std::string pick = env ? env : "auto";
     // dense records: split-FP16 matrix cores unless every entry is 0 / +-1 (then FP32 matrix cores,
@@ -448,6 +449,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     // the dense-op costs and the block width
     const char *fenv = std::getenv("SVX_FUSE");
     const std::string fuse_mode = fenv ? fenv : "auto";
+    // tile scaling for split-FP16 dense groups in the single-buffer kernel (SVX_TILE_SCALE=off: per-column scaling)
+    const char *tenv = std::getenv("SVX_TILE_SCALE");
+    const bool tile_scale_ok = !(tenv && std::string(tenv) == "off");
     FuseCfg fcfg;
     if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
         double c4 = fcfg.cost4, c5 = fcfg.cost5;
@@ -534,18 +538,19 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             cost += kOpCost[od.kind];
             pops.push_back(od);
         }
-        if (dtype == SVX_C64 && !small && fuse_mode != "off" && (pick == "auto" || pick == "v1")) {
+        if (dtype == SVX_C64 && !small && fuse_mode != "off" &&
+            (pick == "auto" || pick == "v1" || pick == "dma" || pick == "dma5")) {
             fuse_pass(pops, dense_of, dus, fcfg);
             for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
         int mode;   // 0 small, 1 single-buffer, 2 DMA, 3 persistent
         if (small) mode = 0;
         else if (pick == "v1" || !dma_ok) mode = 1;
-        else if (has_dense) mode = pick == "dma5" ? 2 : 1;   // dense groups: the single-buffer kernel
+        else if (has_dense) mode = (pick == "dma5" || pick == "dma") ? 2 : 1;   // dense groups: single-buffer kernel
         else if (pick == "pers") mode = 3;
         else if (pick == "dma" || pick == "dma5") mode = 2;
-        else mode = cost <= kDmaMaxCost ? 2 : 1;
-        const bool dma5 = mode == 2 && k == 13 && (pick == "dma5" || has_dense);
+        else mode = 1;   // light and heavy passes alike: the single-buffer kernel streams at ~6 TB/s
+        const bool dma5 = mode == 2 && k == 13 && pick == "dma5";
         const int K = mode == 0 ? nbits : (mode == 1 ? Kv1 : k);
         const int R = mode == 2 ? (dma5 ? 5 : 4) : (mode == 3 ? (k == 12 ? 4 : 5) : kR);
         PassDev pd{};
@@ -621,6 +626,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         }
         // groups: maximal runs of ops whose targets fit R register bits; a dense op is a group of its own
         size_t i = 0;
+        double log2_growth = 0;   // bound on the growth of the tile's largest |amplitude| over the pass
         while (i < pops.size()) {
             if (dense_of[i] >= 0) {
                 const DenseUnit &du = dus[dense_of[i]];
@@ -659,10 +665,17 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                         for (int st = 0; st < D; ++st)
                             for (int l = 0; l < 64; ++l) wmat.push_back((float)W(32 * rt + l % 32, 2 * st + l / 32));
                 } else {
-                    // split-FP16 A operands of v_mfma_f32_32x32x16_f16: 2^14 W = Wh + Wl; block
-                    // ((rt NKS + ks) 2 + part) holds lane l, entry e = W[32 rt + l % 32][16 ks + 8 (l / 32) + e]
+                    // split-FP16 A operands of v_mfma_f32_32x32x16_f16: 2^wexp W = Wh + Wl with the
+                    // largest |entry| of 2^wexp W below 2^14; block ((rt NKS + ks) 2 + part) holds
+                    // lane l, entry e = W[32 rt + l % 32][16 ks + 8 (l / 32) + e]
                     gd.type = 3;
                     const int NKS = D / 8;
+                    double wmax = 0;
+                    for (double v : du.u) wmax = std::max(wmax, std::fabs(v));
+                    int wE = 0;
+                    if (wmax > 0) std::frexp(wmax, &wE);   // wmax < 2^wE
+                    gd.wexp = wmax > 0 ? 14 - wE : 0;
+                    const double wsc = std::ldexp(1.0, gd.wexp);
                     std::vector<uint16_t> hb;
                     hb.reserve((size_t)NRT * NKS * 2 * 512);
                     for (int rt = 0; rt < NRT; ++rt)
@@ -670,7 +683,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                             for (int part = 0; part < 2; ++part)
                                 for (int l = 0; l < 64; ++l)
                                     for (int e = 0; e < 8; ++e) {
-                                        const double ws = W(32 * rt + l % 32, 16 * ks + 8 * (l / 32) + e) * 16384.0;
+                                        const double ws = W(32 * rt + l % 32, 16 * ks + 8 * (l / 32) + e) * wsc;
                                         const uint16_t hh = f32_to_f16_bits((float)ws);
                                         hb.push_back(part == 0 ? hh : f32_to_f16_bits((float)(ws - f16_bits_to_f32(hh))));
                                     }
@@ -678,6 +691,14 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     wmat.resize(o + hb.size() / 2);
                     std::memcpy(wmat.data() + o, hb.data(), hb.size() * sizeof(uint16_t));
                 }
+                // max-abs growth bound of the op: largest row sum of |U| (padding adds identity rows)
+                double rs_max = 0;
+                for (int r = 0; r < d0; ++r) {
+                    double rs = 0;
+                    for (int c = 0; c < d0; ++c) rs += std::hypot(du.u[2 * (r * d0 + c)], du.u[2 * (r * d0 + c) + 1]);
+                    rs_max = std::max(rs_max, rs);
+                }
+                log2_growth += std::log2(std::max(1.0, rs_max));
                 groups.push_back(gd);
                 ++i;
                 continue;
@@ -723,9 +744,26 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     else if (od.kind == K_CPHASE) od.kind = K_CPHASE1;
                 }
                 ops.push_back(od);
+                auto rows = [](const T *mm) {
+                    const double a = std::hypot((double)mm[0], (double)mm[1]) + std::hypot((double)mm[2], (double)mm[3]);
+                    const double b = std::hypot((double)mm[4], (double)mm[5]) + std::hypot((double)mm[6], (double)mm[7]);
+                    return std::max(a, b);
+                };
+                log2_growth += std::log2(std::max(1.0, std::max(rows(od.m0), rows(od.m1))));
             }
             groups.push_back(gd);
             i = j;
+        }
+        // Tile scaling for the split-FP16 groups: the kernel scales each tile so its largest
+        // component is below 2^tscale; 2^tscale times the pass's growth bound (and sqrt 2 for
+        // re/im against complex magnitudes) stays below 2^15, the FP16 range the split needs.
+        if (mode == 1 && has_dense && tile_scale_ok) {
+            const int h = (int)std::ceil(log2_growth + 0.5 + 0.02);
+            if (15 - h >= 4) {
+                pd.tscale = 15 - h;
+                for (int g = pd.group0; g < (int)groups.size(); ++g)
+                    if (groups[g].type == 3) groups[g].type = 4;
+            }
         }
         pd.ngroups = (int)groups.size() - pd.group0;
         LaunchInfo &lr = launches.back();
@@ -771,8 +809,9 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     const char *denv = std::getenv("SVX_DENSE_MATH");
     const char *fenv = std::getenv("SVX_FUSE");
     const char *cenv = std::getenv("SVX_FUSE_COST");
+    const char *tenv = std::getenv("SVX_TILE_SCALE");
     const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
-                             (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "");
+                             (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);

@@ -13,7 +13,6 @@ constexpr int kMaxKc64 = 14;     // 2^14 complex64 = 128 KiB (+ padding) of LDS
 constexpr int kMaxKc128 = 13;    // 2^13 complex128 = 128 KiB (+ padding)
 constexpr int kSmallMaxBits = 9; // states of <= 2^9 amplitudes run in one workgroup (k_pass_small)
 constexpr int kMaxTileSub = 8;   // at most 2^8 QSV2 tiles per workgroup tile
-constexpr double kDmaMaxCost = 40; // passes up to ~5 general 2x2 ops (VALU FMA per amplitude) use k_pass_dma
 
 // op kinds (host classification of the MUX2 record; exact comparisons on the typed values)
 enum OpKind : int32_t {
@@ -35,6 +34,9 @@ struct PassDev {
     int32_t ngroups, group0, nops, op0;
     int32_t h[kMaxM];            // physical bit of tile qubit k-m+i
     int32_t hs[kMaxM];           // h sorted ascending (tile-id deposit)
+    int32_t tscale;              // > 0: scale each LDS tile so its largest component is < 2^tscale
+                                 // (dense groups of type 4 rely on it); 0: no tile scaling
+    int32_t pad0;
     uint64_t dma_u[64];          // DMA kernel: element offset of tile elements 128 j .. from bits >= 7
 };
 
@@ -42,10 +44,14 @@ struct GroupDev {
     int32_t nops, op0;
     int32_t b[kR];               // workgroup-tile bit held by register bit q
     int32_t bs[kR];              // b sorted ascending (slot deposit)
-    int32_t type;                // 0: MUX2 register group; 1: dense op, MFMA; 2: dense op, small-state kernel
+    int32_t type;                // 0: MUX2 register group; 1: dense op, FP32 MFMA; 2: dense op, small-state
+                                 // kernel; 3: dense op, split-FP16 MFMA, per-column scaling; 4: the same
+                                 // on a tile pre-scaled by the pass kernel (PassDev::tscale)
     int32_t f;                   // dense: qubits of the op (type 1: 4 or 5 after padding)
     int32_t dq[5];               // dense: workgroup-tile bit of the op's qubit i (matrix index bit i)
     int32_t dqs[5];              // dq sorted ascending (column deposit)
+    int32_t wexp;                // split-FP16 dense: the matrix was scaled by 2^wexp before the split
+    int32_t pad1;
     int64_t w_off;               // dense: float offset of the matrix (type 1: MFMA A-operand order;
                                  // type 2: row-major re, im)
 };

@@ -235,9 +235,30 @@ __device__ __forceinline__ void apply_op(C2<T> (&a)[1 << R], const OpDev<T> &op,
 // ------------------------------------------------------------------------------------------ pass
 // HBM <-> LDS for one workgroup tile.  Element e of the workgroup tile: QSV2 tile u = e >> k,
 // tile index j = e & (2^k - 1); its state index is tb[u] | sbo[j >> (k-m)] | (j & low mask).
+#ifndef SVX_TILE_NT
+#define SVX_TILE_NT 1
+#endif
+typedef float f4v __attribute__((ext_vector_type(4)));
+// 16-byte state access that streams past the caches (the state is touched once per sweep)
+__device__ __forceinline__ float4 ld_stream(const float *p) {
+#if SVX_TILE_NT
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(p));
+    return make_float4(v[0], v[1], v[2], v[3]);
+#else
+    return *reinterpret_cast<const float4 *>(p);
+#endif
+}
+__device__ __forceinline__ void st_stream(float *p, float4 v) {
+#if SVX_TILE_NT
+    __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(p));
+#else
+    *reinterpret_cast<float4 *>(p) = v;
+#endif
+}
+
 template <typename T, int K, bool LOAD>
 __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, const uint64_t *sbo,
-                                          const uint64_t *tb, int k, int lowb, uint32_t tid) {
+                                          const uint64_t *tb, int k, int lowb, uint32_t tid, int unscale = 0) {
     constexpr uint32_t NE = 1u << K;
     const uint32_t lowmask = (1u << lowb) - 1u, tmask = (1u << k) - 1u;
     if (sizeof(T) == 4 && lowb >= 1) {
@@ -257,7 +278,7 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
             if (LOAD) {
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i)
-                    v[i] = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(st) + 2 * g[i]);
+                    v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * g[i]);
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i) {
                     const uint32_t e = 2u * ((b0 + i) * kNT + tid), pe = lpad(e);
@@ -270,10 +291,16 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
                     const uint32_t e = 2u * ((b0 + i) * kNT + tid), pe = lpad(e);
                     const C2<T> p0 = tile[pe], p1 = tile[pe + 1];
                     v[i] = make_float4(float(p0.x), float(p0.y), float(p1.x), float(p1.y));
+                    if (unscale) {
+                        v[i].x = ldexpf(v[i].x, unscale);
+                        v[i].y = ldexpf(v[i].y, unscale);
+                        v[i].z = ldexpf(v[i].z, unscale);
+                        v[i].w = ldexpf(v[i].w, unscale);
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i)
-                    *reinterpret_cast<float4 *>(reinterpret_cast<float *>(st) + 2 * g[i]) = v[i];
+                    st_stream(reinterpret_cast<float *>(st) + 2 * g[i], v[i]);
             }
         }
     } else {
@@ -296,12 +323,87 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
                 for (int i = 0; i < BATCH; ++i) tile[lpad((b0 + i) * kNT + tid)] = v[i];
             } else {
 #pragma unroll
-                for (int i = 0; i < BATCH; ++i) v[i] = tile[lpad((b0 + i) * kNT + tid)];
+                for (int i = 0; i < BATCH; ++i) {
+                    v[i] = tile[lpad((b0 + i) * kNT + tid)];
+                    if (unscale) {
+                        v[i].x = T(ldexpf(float(v[i].x), unscale));
+                        v[i].y = T(ldexpf(float(v[i].y), unscale));
+                    }
+                }
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i) st[g[i]] = v[i];
             }
         }
     }
+}
+
+// |x| as an ordered integer (non-negative floats order like their bit patterns)
+__device__ __forceinline__ uint32_t absbits(float x) { return __float_as_uint(x) & 0x7fffffffu; }
+
+// Workgroup maximum of mb (kNT threads, red: kNT / 64 words of LDS) -> shift that brings the tile's
+// largest component below 2^target (0 for an all-zero tile).
+__device__ __forceinline__ int tile_shift(uint32_t mb, uint32_t *red, uint32_t tid, int target) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mb = max(mb, (uint32_t)__shfl_xor((int)mb, off));
+    if ((tid & 63u) == 0) red[tid >> 6] = mb;
+    __syncthreads();
+    uint32_t m = red[0];
+#pragma unroll
+    for (int w = 1; w < kNT / 64; ++w) m = max(m, red[w]);
+    if (m == 0) return 0;
+    const int efield = (int)(m >> 23);
+    const int ex = efield == 0 ? -125 : efield - 126;   // largest component < 2^ex
+    return target - ex;
+}
+
+// Complex64 tile load for passes with pre-scaled dense groups: the tile lands in LDS multiplied by
+// 2^shift so that its largest component is below 2^target; returns the shift.  With K <= 13 and
+// at least one contiguous low bit every element of the thread is loaded into registers first
+// (all loads in flight at once), otherwise the tile is loaded and then rescaled in LDS.
+template <int K>
+__device__ __forceinline__ int load_tile_scaled(C2<float> *__restrict__ st, C2<float> *tile, const uint64_t *sbo,
+                                                const uint64_t *tb, int k, int lowb, uint32_t tid, uint32_t *red,
+                                                int target) {
+    constexpr uint32_t NE = 1u << K;
+    const uint32_t lowmask = (1u << lowb) - 1u, tmask = (1u << k) - 1u;
+    uint32_t mb = 0;
+    if constexpr (K <= 13) {
+        if (lowb >= 1) {
+            constexpr int IT = NE / (2 * kNT);
+            float4 v[IT];
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const uint32_t e = 2u * (i * kNT + tid);
+                const uint32_t j = e & tmask, u = e >> k;
+                const uint64_t g = tb[u] | sbo[j >> lowb] | (j & lowmask);
+                v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * g);
+            }
+#pragma unroll
+            for (int i = 0; i < IT; ++i)
+                mb = max(mb, max(max(absbits(v[i].x), absbits(v[i].y)), max(absbits(v[i].z), absbits(v[i].w))));
+            const int sh = tile_shift(mb, red, tid, target);
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const uint32_t pe = lpad(2u * (i * kNT + tid));
+                tile[pe] = C2<float>{ldexpf(v[i].x, sh), ldexpf(v[i].y, sh)};
+                tile[pe + 1] = C2<float>{ldexpf(v[i].z, sh), ldexpf(v[i].w, sh)};
+            }
+            return sh;
+        }
+    }
+    move_tile<float, K, true>(st, tile, sbo, tb, k, lowb, tid);
+    __syncthreads();
+    for (uint32_t e = tid; e < NE; e += kNT) {
+        const C2<float> x = tile[lpad(e)];
+        mb = max(mb, max(absbits(x.x), absbits(x.y)));
+    }
+    const int sh = tile_shift(mb, red, tid, target);
+    if (sh != 0)
+        for (uint32_t e = tid; e < NE; e += kNT) {
+            const C2<float> x = tile[lpad(e)];
+            tile[lpad(e)] = C2<float>{ldexpf(x.x, sh), ldexpf(x.y, sh)};
+        }
+    return sh;
 }
 
 // ------------------------------------------------------------------------------------------ dense op
@@ -404,16 +506,17 @@ __device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &
     }
 }
 
-// Split-FP16 matrix cores: W (pre-scaled by 2^14 and split W = Wh + Wl on the host) times a column
-// block x scaled by a per-column power of two s (max |x s| < 2^15) and split x s = xh + xl in FP16;
-// y = (Wh xh + Wh xl + Wl xh) / (2^14 s) with FP32 accumulation.  The dropped product and the
+// Split-FP16 matrix cores: W (scaled on the host by 2^wexp so its largest entry is below 2^14, and
+// split W 2^wexp = Wh + Wl) times a column block x scaled by a per-column power of two s (max
+// |x s| < 2^15) and split x s = xh + xl in FP16; y = (Wh xh + Wh xl + Wl xh) / (2^wexp s) with FP32
+// accumulation.  The dropped product and the
 // two-piece representations leave a relative error of about 2^-21 of the column's largest entry
 // per product.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B)
 // l % 32 and K entries 8 (l / 32) + 0..7 of each 16-deep K step.
 // v rounded to 11 significant bits, to nearest with ties away from zero (a carry moves into the exponent)
 __device__ __forceinline__ float round11(float v) { return __uint_as_float((__float_as_uint(v) + 0x1000u) & 0xffffe000u); }
 
-template <int K, int F, int NWAVE, bool SWZ>
+template <int K, int F, int NWAVE, bool SWZ, bool PRE>
 __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                               uint32_t wt) {
     constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
@@ -439,18 +542,21 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
         for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
         const uint32_t pb = TileAddr<SWZ>::of(base);
         C2<float> x[NG][8];
-        float mx = 0.f;
 #pragma unroll
         for (int g = 0; g < NG; ++g)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                x[g][e] = ldc(buf, TileAddr<SWZ>::comb(pb, coff(16u * g + 8u * h + e)));
-                mx = fmaxf(mx, fmaxf(fabsf(x[g][e].x), fabsf(x[g][e].y)));
-            }
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        int ex = 0;
-        frexpf(mx, &ex);                           // mx < 2^ex
-        const int sh = mx > 0.f ? 15 - ex : 0;
+            for (int e = 0; e < 8; ++e) x[g][e] = ldc(buf, TileAddr<SWZ>::comb(pb, coff(16u * g + 8u * h + e)));
+        int sh = 0;   // pre-scaled tile: every component is already below 2^15
+        if constexpr (!PRE) {
+            uint32_t mb = 0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) mb = max(mb, max(absbits(x[g][e].x), absbits(x[g][e].y)));
+            mb = max(mb, (uint32_t)__shfl_xor((int)mb, 32));
+            const int efield = (int)(mb >> 23);
+            sh = mb == 0 ? 0 : 15 - (efield == 0 ? -125 : efield - 126);   // column maximum < 2^15
+        }
         // x s = xh + xl: xh = x s rounded to 11 significant bits (nearest, ties away from zero, by
         // an integer add and mask; an exact FP16 value down to 2^-14, i.e. 2^-28 of the column
         // maximum), xl = x s - xh (exact in FP32) converted toward zero.  xl's sign does not follow
@@ -464,8 +570,8 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
                 float xs[2][2], xh[2][2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    xs[0][u] = ldexpf(x[g][e + u].x, sh);
-                    xs[1][u] = ldexpf(x[g][e + u].y, sh);
+                    xs[0][u] = PRE ? x[g][e + u].x : ldexpf(x[g][e + u].x, sh);
+                    xs[1][u] = PRE ? x[g][e + u].y : ldexpf(x[g][e + u].y, sh);
 #pragma unroll
                     for (int p = 0; p < 2; ++p) xh[p][u] = round11(xs[p][u]);
                 }
@@ -492,7 +598,7 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
             for (int ks = 0; ks < NKS; ++ks)
                 acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bh[ks], acc[rt], 0, 0, 0);
         }
-        const int back = -(sh + 14);
+        const int back = -(sh + G.wexp);
         if (F == 4) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
@@ -511,16 +617,20 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
     }
 }
 
-// One dense group (type 1: FP32 matrix cores, type 3: split FP16) by NWAVE wavefronts
+// One dense group (type 1: FP32 matrix cores, 3: split FP16 with per-column scaling, 4: split FP16
+// on a pre-scaled tile) by NWAVE wavefronts
 template <int K, int NWAVE, bool SWZ>
 __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                             uint32_t wt) {
     if (G.type == 1) {
         if (G.f == 5) dense_apply<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
         else dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
+    } else if (G.type == 4 && !SWZ) {
+        if (G.f == 5) dense_apply_h<K, 5, NWAVE, SWZ, true>(buf, G, wd, wt);
+        else dense_apply_h<K, 4, NWAVE, SWZ, true>(buf, G, wd, wt);
     } else {
-        if (G.f == 5) dense_apply_h<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
-        else dense_apply_h<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
+        if (G.f == 5) dense_apply_h<K, 5, NWAVE, SWZ, false>(buf, G, wd, wt);
+        else dense_apply_h<K, 4, NWAVE, SWZ, false>(buf, G, wd, wt);
     }
 }
 
@@ -551,7 +661,16 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
     }
     __syncthreads();
     const int lowb = k - m;
-    move_tile<T, K, true>(st, tile, sbo, tb, k, lowb, tid);
+    int tsh = 0;   // tile scaling shift (passes with pre-scaled dense groups)
+    if constexpr (DN && sizeof(T) == 4) {
+        if (pd->tscale > 0)
+            tsh = load_tile_scaled<K>(reinterpret_cast<C2<float> *>(st), reinterpret_cast<C2<float> *>(tile), sbo, tb,
+                                      k, lowb, tid, reinterpret_cast<uint32_t *>(tb + (1u << sub)), pd->tscale);
+        else
+            move_tile<T, K, true>(st, tile, sbo, tb, k, lowb, tid);
+    } else {
+        move_tile<T, K, true>(st, tile, sbo, tb, k, lowb, tid);
+    }
     __syncthreads();
 
     constexpr int NR = 1 << kR;
@@ -595,7 +714,7 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
         }
         __syncthreads();
     }
-    move_tile<T, K, false>(st, tile, sbo, tb, k, lowb, tid);
+    move_tile<T, K, false>(st, tile, sbo, tb, k, lowb, tid, -tsh);
 }
 
 // ------------------------------------------------------------------------------------------ pass, DMA
@@ -730,7 +849,7 @@ __global__ __launch_bounds__(64 + NW) void k_pass_dma(C2<float> *__restrict__ st
                 const uint32_t e = 2u * (S ^ ((S >> 4) & 15u));
                 const uint64_t gi = tb | sbo[e >> lowb] | (e & lowmask);
                 const float4 v = *reinterpret_cast<const float4 *>(buf + 16u * S);
-                *reinterpret_cast<float4 *>(base + 2 * gi) = v;
+                st_stream(base + 2 * gi, v);
             }
             cur ^= 1;
         }
@@ -846,6 +965,7 @@ hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *g
     if (wd) {   // pass with dense groups
         switch (K) {
         case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
+        case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512, true>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, wd, ntiles); break;
         case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
         default: return hipErrorInvalidValue;
         }
@@ -943,7 +1063,7 @@ __global__ __launch_bounds__(kNT) void k_pass_small(C2<T> *__restrict__ st, cons
 size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits) {
     if (small) return 0;
     const size_t nep = (size_t(1) << K) + ((size_t(1) << K) >> 5);
-    return nep * elem_bytes + ((size_t(1) << m) + (size_t(1) << (K - k))) * sizeof(uint64_t);
+    return nep * elem_bytes + ((size_t(1) << m) + (size_t(1) << (K - k))) * sizeof(uint64_t) + 64;
 }
 
 template <typename T>
@@ -1318,7 +1438,7 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<double, 13, false>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
     SVX_ATTR((k_pass_dma<12, 4, 256, false>)) SVX_ATTR((k_pass_dma<13, 4, 512, false>))
     SVX_ATTR((k_pass_dma<13, 5, 256, false>)) SVX_ATTR((k_pass_dma<12, 4, 256, true>))
-    SVX_ATTR((k_pass_dma<13, 5, 256, true>))
+    SVX_ATTR((k_pass_dma<13, 5, 256, true>)) SVX_ATTR((k_pass_dma<13, 4, 512, true>))
     SVX_ATTR((k_pass_pers<12, 4, 256>)) SVX_ATTR((k_pass_pers<13, 5, 256>))
 #undef SVX_ATTR
     return e;
