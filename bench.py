@@ -187,6 +187,33 @@ def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
     return out
 
 
+def served_section(serving, torch, seconds=20.0, seed=3, burst=60):
+    """Served mix: capacity from a calibration burst (requests all at time 0, another seed), then a
+    Poisson stream at 75% of it for `seconds`; circuits/s, p50/p99 latency (arrival to FP64
+    marginals on the host), and the complex128 plain fallback as judge on a sample."""
+    cache = {}
+    t0 = time.perf_counter()
+    cal = serving.make_stream(n_requests=burst, seed=seed + 1000, compile_cache=cache)
+    srv = serving.SVServer()
+    done = srv.serve(cal, realtime=False)
+    cap = serving.latency_stats(done)["circuits_per_s"]
+    rate = 0.75 * cap
+    reqs = serving.make_stream(rate=rate, duration=seconds, seed=seed, compile_cache=cache)
+    t_compile = time.perf_counter() - t0
+    srv = serving.SVServer()
+    done = srv.serve(reqs, realtime=True)
+    st = serving.latency_stats(done)
+    judge = serving.judge_c128(done, every=10, max_qubits=24)
+    big = [r for r in done if r.n >= 30]
+    return {"workload": f"Poisson stream, seed {seed}, {seconds:.0f} s at {rate:.2f}/s (75% of the {cap:.2f}/s "
+                        f"measured on a {burst}-request burst); n ~ U{{12..28}} w.p. 0.95, U{{30..32}} w.p. 0.05; "
+                        f"depth ~ U{{10..60}}; marginals of logical qubits 0..7 in FP64",
+            "rate_per_s": round(rate, 3), "capacity_burst_per_s": cap, **st,
+            "batched_steps": srv.n_steps, "mean_items_per_step": round(srv.n_launch_items / max(1, srv.n_steps), 2),
+            "graph_requests": len(big), "peak_resident_gib": round(srv.peak_in_use / 2 ** 30, 2),
+            "judge_c128_sampled": judge, "client_compile_s": round(t_compile, 1)}
+
+
 def _run_cpu_child():
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"],
                        capture_output=True, text=True, timeout=600)
@@ -203,6 +230,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-judge", action="store_true")
     ap.add_argument("--no-fused", action="store_true", help="skip the 32/34-qubit fused-vs-plain section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-mix section")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-baseline-child", action="store_true")
     a = ap.parse_args()
     if a.cpu_baseline_child:
@@ -216,7 +245,7 @@ def main():
     import torch
     import __graft_entry__ as ge
     ge.load_package()
-    from u55c_statevector_30q_amd import circuit, compiler, libsvx, runtime
+    from u55c_statevector_30q_amd import circuit, compiler, libsvx, runtime, serving
 
     caps = libsvx.init(0)
     n = 30
@@ -282,12 +311,13 @@ def main():
 
     ladder = None if a.no_judge else cp_ladder_section(libsvx, circuit, compiler, torch, np)
     arch = caps.arch.decode().split(":")[0]
+    del state
+    torch.cuda.empty_cache()
     fused = None
     if not a.no_fused:
-        del state
-        torch.cuda.empty_cache()
         fused = {str(nq): fused_section(libsvx, circuit, runtime, torch, nq, arch, judge=(nq == 32 and not a.no_judge))
                  for nq in (32, 34)}
+    served = None if a.no_serve else served_section(serving, torch, seconds=a.serve_seconds)
     cpu = None if a.no_cpu else _run_cpu_child()
     out = {
         "metric": "30q c64 pass GB/s (% of 8 TB/s); fused vs plain time; served circuits/s+p99",
@@ -320,6 +350,7 @@ def main():
         "judge_vs_c128": judge,
         "cp_ladder_30q": ladder,
         "fused_vs_plain": fused,
+        "served_mix": served,
         "cpu_baseline": cpu,
         "device": caps.arch.decode(),
     }

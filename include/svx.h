@@ -69,6 +69,14 @@ int svx_engine(void *st, int64_t ins_off, void *stream);
 int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle);
 /* Launch a prepared segment: kernels only, no allocation or copy (safe under stream capture). */
 int svx_run_prepared(void *state, int64_t handle, void *stream);
+/* Continuous batching: run pass passes[i] of prepared segment handles[i] on states[i] (complex64,
+ * i < n_items <= 4096) with one launch per kernel shape, the items' tiles side by side in one grid.
+ * Each state gets the same arithmetic as in svx_run_prepared, so results are bit-identical alone and
+ * in a batch.  Items must touch distinct states.  Asynchronous on `stream`.  No reference
+ * counterpart (the reference runs one circuit at a time, host/sv/runtime.py:286-334). */
+int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *passes, int32_t n_items,
+                  void *stream);
+
 /* Number of passes (kernel launches) of a prepared segment, and the kernel index of pass p. */
 int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches);
 

@@ -1,0 +1,116 @@
+"""Served mix: continuous batching (svx_run_batch), HIP-graph replay of large circuits and
+admission by footprint.  Each result must be bit-identical alone and in the mix (the batched
+launch runs the same device code per state), and match the complex128 golden model within the
+complex64 judge tolerances."""
+import numpy as np
+import pytest
+
+from oracle import judge, model
+from u55c_statevector_30q_amd import circuit as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(spec, arrivals=None):
+    from u55c_statevector_30q_amd import serving
+    reqs = []
+    for rid, (n, depth) in enumerate(spec):
+        qc = C.u3_cx_layers(n, depth, 500 + rid)
+        prog, fmap, st = serving.compile_request(qc)
+        reqs.append(serving.Request(rid=rid, n=n, prog=prog, fmap=fmap, depth=depth,
+                                    arrival=0.0 if arrivals is None else arrivals[rid]))
+    return reqs
+
+
+def _alone(req):
+    """The same prepared records run on their own, pass by pass, on a fresh state."""
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    s = torch.empty(1 << req.n, dtype=torch.complex64, device="cuda")
+    libsvx.zero_state(s)
+    for _, w in req.prog:
+        libsvx.run_prepared(s, libsvx.prepare(w, libsvx.C64, req.n))
+    torch.cuda.synchronize()
+    return s
+
+
+SPEC = [(12, 20), (13, 9), (14, 6), (16, 11), (12, 7), (18, 8), (20, 5), (13, 30), (15, 12), (17, 10),
+        (19, 7), (12, 60)]
+
+
+@pytest.mark.parametrize("realtime", [False, True])
+def test_alone_vs_mix_bit_identical(gpu, realtime):
+    from u55c_statevector_30q_amd import libsvx, serving
+    arrivals = [0.0005 * i for i in range(len(SPEC))] if realtime else None
+    reqs = _requests(SPEC, arrivals)
+    srv = serving.SVServer(keep_states=True, graph_min_qubits=64)
+    done = srv.serve(reqs, realtime=realtime)
+    assert len(done) == len(SPEC) and srv.n_steps >= 1
+    if not realtime:
+        assert srv.n_launch_items > srv.n_steps      # several circuits shared launches
+    for r in done:
+        alone = _alone(r)
+        assert bool((alone == r.state).all()), (r.rid, r.n)
+        norm, marg = libsvx.reduce_f64(alone, list(r.fmap[:serving.N_MARGINAL]))
+        assert np.array_equal(marg, r.marginals)
+        assert abs(r.norm - 1) < 1e-5
+
+
+def test_mix_matches_golden_model(gpu):
+    from u55c_statevector_30q_amd import serving
+    reqs = _requests([(12, 15), (14, 8), (16, 6), (13, 40)])
+    done = serving.SVServer(keep_states=True).serve(reqs, realtime=False)
+    for r in done:
+        gold = model.run_program(r.prog, r.n, 1, r.fmap, ctype=np.complex128)
+        got = model.unpermute(r.state.cpu().numpy(), r.n, r.fmap)
+        m = judge.metrics(gold, got)
+        assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 2e-5, (r.n, m)
+        p = np.abs(gold.reshape(-1, 1 << serving.N_MARGINAL)) ** 2   # logical qubits 0..7 are the low bits
+        assert np.max(np.abs(p.sum(axis=0) - r.marginals)) < 1e-6
+
+
+def test_graph_replay_matches_eager(gpu):
+    """Large-circuit path: the whole pass sequence captured as a HIP graph (zeroing by a fill kernel,
+    no memset nodes) and replayed several times with eager work in between; every replay equals the
+    eager run bit for bit, also when served next to batched circuits."""
+    import torch
+    from u55c_statevector_30q_amd import libsvx, serving
+    (req,) = _requests([(18, 9)])
+    eager = _alone(req)
+    hs = [libsvx.prepare(w, libsvx.C64, req.n) for _, w in req.prog]
+    st = torch.empty(1 << req.n, dtype=torch.complex64, device="cuda")
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=torch.cuda.Stream()):
+        s = torch.cuda.current_stream().cuda_stream
+        libsvx.zero_state(st, stream=s)
+        for h in hs:
+            libsvx.run_prepared(st, h, stream=s)
+    other = torch.empty(1 << 16, dtype=torch.complex64, device="cuda")
+    (oreq,) = _requests([(16, 5)])
+    for rep in range(4):
+        g.replay()
+        torch.cuda.synchronize()
+        assert bool((st == eager).all()), rep
+        libsvx.zero_state(other)
+        for _, w in oreq.prog:
+            libsvx.run_segment(other, w)
+        st.zero_()
+        torch.cuda.synchronize()
+    reqs = _requests([(18, 9), (12, 10), (14, 7), (16, 8)])
+    srv = serving.SVServer(keep_states=True, graph_min_qubits=18)
+    done = {r.rid: r for r in srv.serve(reqs, realtime=False)}
+    assert done[0].mode == "graph" and bool((done[0].state == eager).all())
+    for rid in (1, 2, 3):
+        assert done[rid].mode == "batch" and bool((done[rid].state == _alone(done[rid])).all())
+
+
+def test_admission_by_footprint(gpu):
+    """With room for three 16-qubit states, no more than three are resident at once and every
+    request still completes."""
+    from u55c_statevector_30q_amd import serving
+    reqs = _requests([(16, 6)] * 8)
+    srv = serving.SVServer(budget_bytes=3 * (1 << 16) * 8)
+    done = srv.serve(reqs, realtime=False)
+    assert len(done) == 8 and srv.peak_in_use <= 3 * (1 << 16) * 8 and srv.in_use == 0
+    with pytest.raises(ValueError):
+        srv.submit(_requests([(18, 2)])[0])

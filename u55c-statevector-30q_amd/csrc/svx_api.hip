@@ -59,6 +59,15 @@ struct DeviceState {
     size_t part_bytes = 0;
     void *misc = nullptr;          // small descriptors / outputs
     size_t misc_bytes = 0;
+    // batched launches: a ring of item tables (pinned host staging + device copy), one event per
+    // slot so a slot is rewritten only after the launch that read it has finished
+    static constexpr int kBatchSlots = 16;
+    static constexpr int kBatchCap = 4096;
+    BatchItem *batch_host = nullptr;
+    BatchItem *batch_dev = nullptr;
+    hipEvent_t batch_ev[kBatchSlots] = {};
+    bool batch_used[kBatchSlots] = {};
+    int batch_next = 0;
     std::mutex mu;
 };
 std::mutex g_mu;
@@ -986,6 +995,79 @@ int svx_run_prepared(void *state, int64_t handle, void *stream) {
         pr = g_prepared[handle - 1].get();
     }
     return run_prepared(state, pr, (hipStream_t)stream);
+}
+
+int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *passes, int32_t n_items,
+                  void *stream) {
+    if (n_items < 0 || (n_items > 0 && (!states || !handles || !passes))) return fail(SVX_E_ARG, "bad batch arrays");
+    if (n_items == 0) return 0;
+    if (n_items > DeviceState::kBatchCap) return fail(SVX_E_ARG, "more than 4096 items in one batch");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    struct Cls {
+        std::vector<BatchItem> items;
+        uint64_t tiles = 0;
+        size_t lds = 0;
+    };
+    std::map<int, Cls> cls;   // key: K * 2 + dense
+    {
+        std::lock_guard<std::mutex> lk(g_prep_mu);
+        for (int i = 0; i < n_items; ++i) {
+            const int64_t h = handles[i];
+            if (h < 1 || h > (int64_t)g_prepared.size() || !g_prepared[h - 1])
+                return fail(SVX_E_ARG, "unknown prepared-segment handle in batch item " + std::to_string(i));
+            Prepared *pr = g_prepared[h - 1].get();
+            if (pr->dtype != SVX_C64) return fail(SVX_E_ARG, "batched launches run complex64 segments");
+            if (pr->device != ds->caps.device) return fail(SVX_E_ARG, "batch item prepared on another device");
+            if (passes[i] < 0 || passes[i] >= (int32_t)pr->launches.size())
+                return fail(SVX_E_ARG, "bad pass index in batch item " + std::to_string(i));
+            if (!states[i] || (reinterpret_cast<uintptr_t>(states[i]) & 15))
+                return fail(SVX_E_ARG, "batch states must be non-null and 16-byte aligned");
+            const LaunchInfo &li = pr->launches[passes[i]];
+            if (li.empty) continue;
+            if (li.small || li.dma || li.pers)
+                return fail(SVX_E_ARG, "batch item " + std::to_string(i) +
+                                           ": only passes of the single-buffer kernel batch (state >= 2^10 amplitudes, "
+                                           "SVX_PASS_KERNEL=auto|v1)");
+            unsigned char *base = static_cast<unsigned char *>(pr->dev);
+            BatchItem it{};
+            it.st = states[i];
+            it.pd = reinterpret_cast<const PassDev *>(base + pr->pass_off) + li.pass_index;
+            it.gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
+            it.od = base + pr->op_off;
+            it.wd = li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr;
+            Cls &c = cls[li.K * 2 + (li.dense ? 1 : 0)];
+            it.tile0 = c.tiles;
+            c.tiles += li.ntiles;
+            c.lds = std::max(c.lds, li.lds);
+            c.items.push_back(it);
+        }
+    }
+    std::lock_guard<std::mutex> lk(ds->mu);
+    if (!ds->batch_host) {
+        HIPCHK(hipHostMalloc((void **)&ds->batch_host, sizeof(BatchItem) * DeviceState::kBatchCap * DeviceState::kBatchSlots));
+        HIPCHK(hipMalloc((void **)&ds->batch_dev, sizeof(BatchItem) * DeviceState::kBatchCap * DeviceState::kBatchSlots));
+        for (int i = 0; i < DeviceState::kBatchSlots; ++i) HIPCHK(hipEventCreateWithFlags(&ds->batch_ev[i], hipEventDisableTiming));
+    }
+    for (auto &kv : cls) {
+        Cls &c = kv.second;
+        if (c.tiles > 0xffffffffull) return fail(SVX_E_CAPACITY, "batch launch with more than 2^32 workgroups");
+        const int slot = ds->batch_next;
+        ds->batch_next = (ds->batch_next + 1) % DeviceState::kBatchSlots;
+        if (ds->batch_used[slot]) HIPCHK(hipEventSynchronize(ds->batch_ev[slot]));
+        BatchItem *hs = ds->batch_host + (size_t)slot * DeviceState::kBatchCap;
+        BatchItem *dv = ds->batch_dev + (size_t)slot * DeviceState::kBatchCap;
+        std::memcpy(hs, c.items.data(), c.items.size() * sizeof(BatchItem));
+        HIPCHK(hipMemcpyAsync(dv, hs, c.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, s));
+        const hipError_t e = launch_pass_batch(kv.first / 2, kv.first & 1, dv, (int)c.items.size(), (uint32_t)c.tiles,
+                                               c.lds, s);
+        if (e != hipSuccess) return hipfail(e, "batched pass launch");
+        HIPCHK(hipEventRecord(ds->batch_ev[slot], s));
+        ds->batch_used[slot] = true;
+    }
+    return 0;
 }
 
 int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches) {

@@ -75,7 +75,19 @@ struct PermDev {
     uint64_t newhi[64];          // new-index offset of new-local index l >> c
 };
 
+// one state's pass in a batched launch (continuous batching)
+struct BatchItem {
+    void *st;                    // complex64 state
+    const PassDev *pd;
+    const GroupDev *gd;
+    const void *od;              // OpDev<float> array of the prepared segment
+    const float *wd;             // dense operand blocks of the prepared segment (or null)
+    uint64_t tile0;              // first workgroup of this item in the launch
+};
+
 // launchers (svx_kernels.hip)
+hipError_t launch_pass_batch(int K, bool dense, const BatchItem *items, int n_items, uint32_t grid, size_t lds,
+                             hipStream_t s);
 hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                            const OpDev<float> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,

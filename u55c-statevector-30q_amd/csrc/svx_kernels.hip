@@ -634,11 +634,11 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
     }
 }
 
+// One workgroup tile of one pass: `tile` selects the 2^(K-k) QSV2 tiles the workgroup owns.
 template <typename T, int K, bool DN>
-__global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
-                                              const GroupDev *__restrict__ gd,
-                                              const OpDev<T> *__restrict__ od, const float *__restrict__ wd) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
+                                          const GroupDev *__restrict__ gd, const OpDev<T> *__restrict__ od,
+                                          const float *__restrict__ wd, uint64_t tile_id, unsigned char *smem) {
     constexpr uint32_t NE = 1u << K;
     constexpr uint32_t NEP = NE + (NE >> 5);
     C2<T> *tile = reinterpret_cast<C2<T> *>(smem);
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
         sbo[s] = o;
     }
     for (uint32_t u = tid; u < (1u << sub); u += kNT) {
-        uint64_t x = ((uint64_t(blockIdx.x) << sub) | u) << (k - m);
+        uint64_t x = ((tile_id << sub) | u) << (k - m);
         for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
         tb[u] = x;
     }
@@ -715,6 +715,33 @@ __global__ __launch_bounds__(kNT, DN ? 2 : 1) void k_pass(C2<T> *__restrict__ st
         __syncthreads();
     }
     move_tile<T, K, false>(st, tile, sbo, tb, k, lowb, tid, -tsh);
+}
+
+template <typename T, int K, bool DN>
+__global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
+                                              const GroupDev *__restrict__ gd,
+                                              const OpDev<T> *__restrict__ od, const float *__restrict__ wd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pass_tile<T, K, DN>(st, pd, gd, od, wd, blockIdx.x, smem);
+}
+
+// Continuous batching: one launch runs one pass of each of several states (the items, one kernel
+// shape per launch); workgroup b runs tile b - items[i].tile0 of the item i whose tile range holds
+// b, with the same device code as k_pass, so every state gets the same arithmetic alone and in
+// the batch.
+template <int K, bool DN>
+__global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass_batch(const BatchItem *__restrict__ items, int n_items) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t b = blockIdx.x;
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {   // last item with tile0 <= b
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchItem &it = items[lo];
+    pass_tile<float, K, DN>(reinterpret_cast<C2<float> *>(it.st), it.pd, it.gd,
+                            reinterpret_cast<const OpDev<float> *>(it.od), it.wd, b - it.tile0, smem);
 }
 
 // ------------------------------------------------------------------------------------------ pass, DMA
@@ -1108,6 +1135,24 @@ hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const
                            const OpDev<float> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s) {
     return launch_pass_t<float>(K, small, st, pd, gd, od, wd, grid, lds, s);
 }
+hipError_t launch_pass_batch(int K, bool dense, const BatchItem *items, int n_items, uint32_t grid, size_t lds,
+                             hipStream_t s) {
+    switch (K * 2 + (dense ? 1 : 0)) {
+    case 20: hipLaunchKernelGGL((k_pass_batch<10, false>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 21: hipLaunchKernelGGL((k_pass_batch<10, true>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 22: hipLaunchKernelGGL((k_pass_batch<11, false>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 23: hipLaunchKernelGGL((k_pass_batch<11, true>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 24: hipLaunchKernelGGL((k_pass_batch<12, false>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 25: hipLaunchKernelGGL((k_pass_batch<12, true>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 26: hipLaunchKernelGGL((k_pass_batch<13, false>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 27: hipLaunchKernelGGL((k_pass_batch<13, true>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 28: hipLaunchKernelGGL((k_pass_batch<14, false>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    case 29: hipLaunchKernelGGL((k_pass_batch<14, true>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                             const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s) {
     return launch_pass_t<double>(K, small, st, pd, gd, od, nullptr, grid, lds, s);
@@ -1434,6 +1479,10 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<float, 13, false>)) SVX_ATTR((k_pass<float, 14, false>))
     SVX_ATTR((k_pass<float, 10, true>)) SVX_ATTR((k_pass<float, 11, true>)) SVX_ATTR((k_pass<float, 12, true>))
     SVX_ATTR((k_pass<float, 13, true>)) SVX_ATTR((k_pass<float, 14, true>))
+    SVX_ATTR((k_pass_batch<10, false>)) SVX_ATTR((k_pass_batch<10, true>)) SVX_ATTR((k_pass_batch<11, false>))
+    SVX_ATTR((k_pass_batch<11, true>)) SVX_ATTR((k_pass_batch<12, false>)) SVX_ATTR((k_pass_batch<12, true>))
+    SVX_ATTR((k_pass_batch<13, false>)) SVX_ATTR((k_pass_batch<13, true>)) SVX_ATTR((k_pass_batch<14, false>))
+    SVX_ATTR((k_pass_batch<14, true>))
     SVX_ATTR((k_pass<double, 10, false>)) SVX_ATTR((k_pass<double, 11, false>)) SVX_ATTR((k_pass<double, 12, false>))
     SVX_ATTR((k_pass<double, 13, false>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
     SVX_ATTR((k_pass_dma<12, 4, 256, false>)) SVX_ATTR((k_pass_dma<13, 4, 512, false>))

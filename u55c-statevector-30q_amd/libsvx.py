@@ -19,7 +19,7 @@ C64, C128 = 0, 1
 SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -1, -2, -3, -4, -5, -6
 
 SYMBOLS = ["svx_init", "svx_run_segment", "svx_engine", "svx_prepare", "svx_run_prepared",
-           "svx_prepared_info", "svx_prepared_kernel", "svx_prepared_pass_stats", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
+           "svx_prepared_info", "svx_prepared_kernel", "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
            "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync", "svx_last_error",
            "svx_version"]
 
@@ -46,6 +46,7 @@ _SIGS = {
     "svx_prepared_info": ([_I64, _PI32, _PI32], _I32),
     "svx_prepared_kernel": ([_I64, _I32, ctypes.c_char_p, _I32], _I32),
     "svx_prepared_pass_stats": ([_I64, _I32, _PI32, ], _I32),
+    "svx_run_batch": ([ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int64), _PI32, _I32, _VP], _I32),
     "svx_permute": ([_VP, _VP, _I32, _PI32, _I32, _VP], _I32),
     "svx_swap_words": ([_VP, _VP, _I64, _I64, _I64, _VP], _I32),
     "svx_zero_state": ([_VP, _I32, _I32, _VP], _I32),
@@ -195,6 +196,22 @@ def prepared_kernel(handle, pass_index):
     buf = ctypes.create_string_buffer(64)
     _check(lib.svx_prepared_kernel(handle, pass_index, buf, 64))
     return buf.value.decode()
+
+
+def run_batch(states, handles, passes, stream=None):
+    """svx_run_batch: one pass of each (state, prepared handle, pass index) item, batched into one
+    launch per kernel shape (complex64 states)."""
+    lib = load()
+    n = len(states)
+    if not (len(handles) == len(passes) == n):
+        raise ValueError("states, handles and passes must have the same length")
+    for t in states:
+        if dtype_code(t) != C64:
+            raise ValueError("batched launches run complex64 states")
+    sp = (ctypes.c_void_p * n)(*[_dev(t) for t in states])
+    hp = (ctypes.c_int64 * n)(*[int(h) for h in handles])
+    pp = (ctypes.c_int32 * n)(*[int(p) for p in passes])
+    _check(lib.svx_run_batch(sp, hp, pp, n, _stream(stream)))
 
 
 def prepared_pass_stats(handle, pass_index):
