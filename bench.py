@@ -214,6 +214,22 @@ def served_section(serving, torch, seconds=20.0, seed=3, burst=60):
             "judge_c128_sampled": judge, "client_compile_s": round(t_compile, 1)}
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/traffic_*.md, scaled by the copy-kernel calibration there), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.md"))):
+        fields = {}
+        with open(path) as f:
+            for line in f:
+                if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "kernel"):
+                    k, v = line.split(":", 1)
+                    fields[k] = v.strip()
+        if fields.get("kernel") == kernel and "traffic_bytes_per_launch" in fields:
+            return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def _run_cpu_child():
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"],
                        capture_output=True, text=True, timeout=600)
@@ -233,6 +249,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix section")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-baseline-child", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: an 8 GiB complex64 copy (svx_convert, calibration) twice, "
+                         "then one step of the 30-qubit workload; prints the kernel names")
     a = ap.parse_args()
     if a.cpu_baseline_child:
         print(json.dumps(cpu_baseline()))
@@ -250,6 +269,21 @@ def main():
     caps = libsvx.init(0)
     n = 30
     qc = circuit.workload_gpu30()
+    if a.traffic_probe:
+        prog, fmap, st = compiler.compile_gpu(qc)
+        hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
+        src = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
+        dst = torch.empty_like(src)
+        libsvx.zero_state(src)
+        for _ in range(2):
+            libsvx.convert(dst, src)
+        for h in hs:
+            libsvx.run_prepared(src, h)
+        torch.cuda.synchronize()
+        names = sorted({libsvx.prepared_kernel(h, p) for h in hs for p in range(libsvx.prepared_info(h)[1])})
+        print(json.dumps({"copy_bytes_read": (1 << n) * 8, "copy_bytes_written": (1 << n) * 8,
+                          "pass_kernels": names, "passes": st.n_passes}))
+        return
     t0 = time.perf_counter()
     prog, fmap, st = compiler.compile_gpu(qc)
     t_compile = time.perf_counter() - t0
@@ -294,6 +328,7 @@ def main():
     achieved = bytes_per_pass / (mean_ms * 1e-3) / 1e9
     per_pass_gbs = [bytes_per_pass / (t * 1e-3) / 1e9 for t in per_pass_ms]
 
+    traffic, traffic_src = profiled_traffic(dominant)
     judge = None
     if not a.no_judge:
         s64 = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
@@ -338,7 +373,7 @@ def main():
                    "tile_k": compiler.GPU_K, "compile_s": round(t_compile, 3)},
         "frac_of_peak": round(value / PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": dominant, "bytes_per_launch": bytes_per_pass,
                      "mean_launch_ms": round(mean_ms, 4), "launches": len(by_kernel[dominant]),
                      "per_kernel": {kk: {"launches": len(v), "mean_ms": round(float(np.mean(v)), 4),
