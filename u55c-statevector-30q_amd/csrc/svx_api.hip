@@ -683,7 +683,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     for (double v : du.u) wmax = std::max(wmax, std::fabs(v));
                     int wE = 0;
                     if (wmax > 0) std::frexp(wmax, &wE);   // wmax < 2^wE
-                    gd.wexp = wmax > 0 ? 14 - wE : 0;
+                    // entries within [2^-4, 2]: no scaling (the low halves' FP16 spacing 2^-24 stays
+                    // below FP32 rounding of the products); otherwise the largest entry goes below 2^14
+                    gd.wexp = (wmax == 0 || (wmax >= 0.0625 && wmax <= 2.0)) ? 0 : 14 - wE;
                     const double wsc = std::ldexp(1.0, gd.wexp);
                     std::vector<uint16_t> hb;
                     hb.reserve((size_t)NRT * NKS * 2 * 512);

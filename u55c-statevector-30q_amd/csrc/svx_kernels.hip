@@ -506,15 +506,15 @@ __device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &
     }
 }
 
-// Split-FP16 matrix cores: W (scaled on the host by 2^wexp so its largest entry is below 2^14, and
-// split W 2^wexp = Wh + Wl) times a column block x scaled by a per-column power of two s (max
-// |x s| < 2^15) and split x s = xh + xl in FP16; y = (Wh xh + Wh xl + Wl xh) / (2^wexp s) with FP32
-// accumulation.  The dropped product and the
+// Split-FP16 matrix cores: W (scaled on the host by 2^wexp -- 0 for entries within [2^-4, 2], else
+// so its largest entry is below 2^14 -- and split W 2^wexp = Wh + Wl) times a column block x scaled
+// by a power of two s (per tile or per column, max |x s| < 2^15) and split x s = xh + xl in FP16;
+// y = (Wh xh + Wh xl + Wl xh) / (2^wexp s) with FP32 accumulation.  The dropped product and the
 // two-piece representations leave a relative error of about 2^-21 of the column's largest entry
 // per product.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B)
 // l % 32 and K entries 8 (l / 32) + 0..7 of each 16-deep K step.
-// v rounded to 11 significant bits, to nearest with ties away from zero (a carry moves into the exponent)
-__device__ __forceinline__ float round11(float v) { return __uint_as_float((__float_as_uint(v) + 0x1000u) & 0xffffe000u); }
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
 template <int K, int F, int NWAVE, bool SWZ, bool PRE>
 __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
@@ -557,11 +557,10 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
             const int efield = (int)(mb >> 23);
             sh = mb == 0 ? 0 : 15 - (efield == 0 ? -125 : efield - 126);   // column maximum < 2^15
         }
-        // x s = xh + xl: xh = x s rounded to 11 significant bits (nearest, ties away from zero, by
-        // an integer add and mask; an exact FP16 value down to 2^-14, i.e. 2^-28 of the column
-        // maximum), xl = x s - xh (exact in FP32) converted toward zero.  xl's sign does not follow
-        // x's, so the truncation does not shrink amplitudes on average and the norm does not drift
-        // over many dense ops.
+        // x s = xh + xl: xh = x s truncated to 11 significant bits (a mask; an exact FP16 value down
+        // to 2^-14, i.e. 2^-28 of the column maximum), xl = x s - xh (exact in FP32) rounded to the
+        // nearest FP16, so the split does not shrink amplitudes on average and the norm does not
+        // drift over many dense ops.
         half8 bh[NKS], bl[NKS];
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -573,12 +572,12 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
                     xs[0][u] = PRE ? x[g][e + u].x : ldexpf(x[g][e + u].x, sh);
                     xs[1][u] = PRE ? x[g][e + u].y : ldexpf(x[g][e + u].y, sh);
 #pragma unroll
-                    for (int p = 0; p < 2; ++p) xh[p][u] = round11(xs[p][u]);
+                    for (int p = 0; p < 2; ++p) xh[p][u] = __uint_as_float(__float_as_uint(xs[p][u]) & 0xffffe000u);
                 }
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
                     const auto hp = __builtin_amdgcn_cvt_pkrtz(xh[p][0], xh[p][1]);
-                    const auto lp = __builtin_amdgcn_cvt_pkrtz(xs[p][0] - xh[p][0], xs[p][1] - xh[p][1]);
+                    const half2v lp = __builtin_convertvector(float2v{xs[p][0] - xh[p][0], xs[p][1] - xh[p][1]}, half2v);
                     bh[p * NG + g][e] = hp[0];
                     bh[p * NG + g][e + 1] = hp[1];
                     bl[p * NG + g][e] = lp[0];
@@ -599,19 +598,23 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
                 acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bh[ks], acc[rt], 0, 0, 0);
         }
         const int back = -(sh + G.wexp);
+        if (back != 0) {
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[rt][v] = ldexpf(acc[rt][v], back);
+        }
         if (F == 4) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)),
-                    C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[0][v + 8], back)});
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[0][v + 8]});
             }
         } else {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)),
-                    C2<float>{ldexpf(acc[0][v], back), ldexpf(acc[NRT - 1][v], back)});
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[NRT - 1][v]});
             }
         }
     }
