@@ -239,7 +239,7 @@ def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m, monkeypatch):
 
 
 @pytest.mark.parametrize("fuse,cost", [("off", None), ("auto", None), ("on", None), ("on", "0,0,5"),
-                                       ("on", "0,0,3"), ("on", "0,0,1")])
+                                       ("on", "0,0,3"), ("on", "0,0,1"), ("dense4", None)])
 @pytest.mark.parametrize("n,k,max_m", [(12, 10, 4), (16, 12, 6), (20, 13, 7), (21, 12, 4), (22, 13, 7)])
 def test_pass_fusion_matches_oracle(gpu, n, k, max_m, fuse, cost, monkeypatch):
     """Pass fusion (MUX2 ops of a pass clustered into dense units on the matrix cores): every
@@ -325,3 +325,24 @@ def test_pass_fusion_double_buffered_kernels(gpu, n, k, pick, monkeypatch):
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
     got = run_hip(s0.astype(np.complex64), seg)
     _check_c64(ref, got, (n, k, pick))
+
+
+@pytest.mark.parametrize("n,k", [(18, 12), (20, 13), (22, 13)])
+def test_dense_only_kernel_matches_general_kernel(gpu, n, k, monkeypatch):
+    """Every op of every pass in a dense 4-qubit unit (SVX_FUSE=dense4) runs on the 512-thread
+    dense-only kernel; it is bit-identical to the general dense kernel (SVX_D4=off: same units, same
+    arithmetic, other thread count) and matches the complex128 golden model."""
+    from u55c_statevector_30q_amd import libsvx
+    monkeypatch.setenv("SVX_FUSE", "dense4")
+    rng = np.random.default_rng(3 * n + k)
+    seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got = {}
+    for d4 in ("auto", "off"):
+        monkeypatch.setenv("SVX_D4", d4)
+        got[d4] = run_hip(s0.astype(np.complex64), seg)
+        _check_c64(ref, got[d4], (n, k, d4))
+    h = libsvx.prepare(seg, libsvx.C64, n)
+    names = [libsvx.prepared_kernel(h, p) for p in range(3)]
+    assert np.array_equal(got["auto"], got["off"]), names

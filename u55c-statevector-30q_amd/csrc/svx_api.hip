@@ -239,6 +239,7 @@ struct LaunchInfo {
     int64_t pass_index;   // index into the PassDev array
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
+    bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
     int32_t stats[5];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32)
 };
 
@@ -279,7 +280,8 @@ struct DenseUnit {
 const double kOpCost[] = {8, 4, 1, 8, 4, 2, 2, 8, 2, 2};
 
 struct FuseCfg {
-    int width = 5;          // largest block (tile bits)
+    bool all_dense = false; // every block (also single ops) becomes a dense unit
+    int width = 4;          // largest block (tile bits): 4-qubit units run on the 512-thread kernel
     double cost4 = 8.0;     // cost of one dense op padded to 4 qubits, same units as kOpCost
     double cost5 = 16.0;    // ... 5 qubits
 };
@@ -401,7 +403,7 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
             for (int i : mem) mux += kOpCost[pops[i].kind];
         const int nq = __builtin_popcount(S);
         const double dcost = std::max(4, nq) == 5 ? cfg.cost5 : cfg.cost4;
-        if (!single && mem.size() >= 2 && mux > dcost) {
+        if (!single && (cfg.all_dense || (mem.size() >= 2 && mux > dcost))) {
             std::vector<int> qs;
             for (int b = 0; b < 32; ++b)
                 if ((S >> b) & 1u) qs.push_back(b);
@@ -462,6 +464,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     const char *tenv = std::getenv("SVX_TILE_SCALE");
     const bool tile_scale_ok = !(tenv && std::string(tenv) == "off");
     FuseCfg fcfg;
+    if (fuse_mode == "dense4") {   // every block of at most 4 tile bits a dense unit
+        fcfg.width = 4;
+        fcfg.all_dense = true;
+    }
+    const char *d4env = std::getenv("SVX_D4");
+    const std::string d4_mode = d4env ? d4env : "auto";
     if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
         double c4 = fcfg.cost4, c5 = fcfg.cost5;
         int wd = fcfg.width;
@@ -778,6 +786,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         }
         pd.ngroups = (int)groups.size() - pd.group0;
         LaunchInfo &lr = launches.back();
+        // dense-only passes of 4-qubit units run on the 512-thread kernel (SVX_D4=off: never)
+        if (mode == 1 && has_dense && d4_mode != "off" && (K == 12 || K == 13) && pd.ngroups > 0) {
+            bool all4 = true;
+            for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].f == 4;
+            lr.d4 = all4;
+        }
         for (int g = pd.group0; g < (int)groups.size(); ++g) {
             const GroupDev &G = groups[g];
             if (G.type == 0) {
@@ -821,8 +835,10 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     const char *fenv = std::getenv("SVX_FUSE");
     const char *cenv = std::getenv("SVX_FUSE_COST");
     const char *tenv = std::getenv("SVX_TILE_SCALE");
+    const char *d4env = std::getenv("SVX_D4");
     const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
-                             (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "");
+                             (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "") + "/" +
+                             (d4env ? d4env : "");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);
@@ -866,7 +882,10 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
     for (const LaunchInfo &li : pr->launches) {
         if (li.empty) continue;
         hipError_t e;
-        if (li.pers)
+        if (li.d4)
+            e = launch_pass_d4(li.K, state, pd + li.pass_index, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
+                               li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr, li.grid, li.lds, s);
+        else if (li.pers)
             e = launch_pass_pers(li.K, state, pd + li.pass_index, gd,
                                  reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
         else if (li.dma)
@@ -1013,7 +1032,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         uint64_t tiles = 0;
         size_t lds = 0;
     };
-    std::map<int, Cls> cls;   // key: K * 2 + dense
+    std::map<int, Cls> cls;   // key: K * 4 + dense + 2 d4
     {
         std::lock_guard<std::mutex> lk(g_prep_mu);
         for (int i = 0; i < n_items; ++i) {
@@ -1040,7 +1059,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
             it.gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
             it.od = base + pr->op_off;
             it.wd = li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr;
-            Cls &c = cls[li.K * 2 + (li.dense ? 1 : 0)];
+            Cls &c = cls[li.K * 4 + (li.dense ? 1 : 0) + (li.d4 ? 2 : 0)];
             it.tile0 = c.tiles;
             c.tiles += li.ntiles;
             c.lds = std::max(c.lds, li.lds);
@@ -1063,8 +1082,8 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         BatchItem *dv = ds->batch_dev + (size_t)slot * DeviceState::kBatchCap;
         std::memcpy(hs, c.items.data(), c.items.size() * sizeof(BatchItem));
         HIPCHK(hipMemcpyAsync(dv, hs, c.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, s));
-        const hipError_t e = launch_pass_batch(kv.first / 2, kv.first & 1, dv, (int)c.items.size(), (uint32_t)c.tiles,
-                                               c.lds, s);
+        const hipError_t e = launch_pass_batch(kv.first / 4, kv.first & 1, (kv.first & 2) != 0, dv, (int)c.items.size(),
+                                               (uint32_t)c.tiles, c.lds, s);
         if (e != hipSuccess) return hipfail(e, "batched pass launch");
         HIPCHK(hipEventRecord(ds->batch_ev[slot], s));
         ds->batch_used[slot] = true;
@@ -1097,6 +1116,7 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
     else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
     else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5" : ",4") + (li.dense ? ",dense>" : ">");
+    else if (li.d4) k = "k_pass_d4<" + std::to_string(li.K) + ">";
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());
     return 0;

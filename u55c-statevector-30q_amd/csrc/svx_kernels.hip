@@ -256,14 +256,14 @@ __device__ __forceinline__ void st_stream(float *p, float4 v) {
 #endif
 }
 
-template <typename T, int K, bool LOAD>
+template <typename T, int K, bool LOAD, int NT = kNT>
 __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, const uint64_t *sbo,
                                           const uint64_t *tb, int k, int lowb, uint32_t tid, int unscale = 0) {
     constexpr uint32_t NE = 1u << K;
     const uint32_t lowmask = (1u << lowb) - 1u, tmask = (1u << k) - 1u;
     if (sizeof(T) == 4 && lowb >= 1) {
         // two adjacent amplitudes (16 bytes) per lane and access
-        constexpr int IT = NE / (2 * kNT);
+        constexpr int IT = NE / (2 * NT);
         constexpr int BATCH = IT < 8 ? IT : 8;
 #pragma unroll 1
         for (int b0 = 0; b0 < IT; b0 += BATCH) {
@@ -271,7 +271,7 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
             uint64_t g[BATCH];
 #pragma unroll
             for (int i = 0; i < BATCH; ++i) {
-                const uint32_t e = 2u * ((b0 + i) * kNT + tid);
+                const uint32_t e = 2u * ((b0 + i) * NT + tid);
                 const uint32_t j = e & tmask, u = e >> k;
                 g[i] = tb[u] | sbo[j >> lowb] | (j & lowmask);
             }
@@ -281,14 +281,14 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
                     v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * g[i]);
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i) {
-                    const uint32_t e = 2u * ((b0 + i) * kNT + tid), pe = lpad(e);
+                    const uint32_t e = 2u * ((b0 + i) * NT + tid), pe = lpad(e);
                     tile[pe] = C2<T>{T(v[i].x), T(v[i].y)};
                     tile[pe + 1] = C2<T>{T(v[i].z), T(v[i].w)};
                 }
             } else {
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i) {
-                    const uint32_t e = 2u * ((b0 + i) * kNT + tid), pe = lpad(e);
+                    const uint32_t e = 2u * ((b0 + i) * NT + tid), pe = lpad(e);
                     const C2<T> p0 = tile[pe], p1 = tile[pe + 1];
                     v[i] = make_float4(float(p0.x), float(p0.y), float(p1.x), float(p1.y));
                     if (unscale) {
@@ -304,7 +304,7 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
             }
         }
     } else {
-        constexpr int IT = NE / kNT;
+        constexpr int IT = NE / NT;
         constexpr int BATCH = IT < 8 ? IT : 8;
 #pragma unroll 1
         for (int b0 = 0; b0 < IT; b0 += BATCH) {
@@ -312,7 +312,7 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
             uint64_t g[BATCH];
 #pragma unroll
             for (int i = 0; i < BATCH; ++i) {
-                const uint32_t e = (b0 + i) * kNT + tid;
+                const uint32_t e = (b0 + i) * NT + tid;
                 const uint32_t j = e & tmask, u = e >> k;
                 g[i] = tb[u] | sbo[j >> lowb] | (j & lowmask);
             }
@@ -320,11 +320,11 @@ __device__ __forceinline__ void move_tile(C2<T> *__restrict__ st, C2<T> *tile, c
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i) v[i] = st[g[i]];
 #pragma unroll
-                for (int i = 0; i < BATCH; ++i) tile[lpad((b0 + i) * kNT + tid)] = v[i];
+                for (int i = 0; i < BATCH; ++i) tile[lpad((b0 + i) * NT + tid)] = v[i];
             } else {
 #pragma unroll
                 for (int i = 0; i < BATCH; ++i) {
-                    v[i] = tile[lpad((b0 + i) * kNT + tid)];
+                    v[i] = tile[lpad((b0 + i) * NT + tid)];
                     if (unscale) {
                         v[i].x = T(ldexpf(float(v[i].x), unscale));
                         v[i].y = T(ldexpf(float(v[i].y), unscale));
@@ -342,6 +342,7 @@ __device__ __forceinline__ uint32_t absbits(float x) { return __float_as_uint(x)
 
 // Workgroup maximum of mb (kNT threads, red: kNT / 64 words of LDS) -> shift that brings the tile's
 // largest component below 2^target (0 for an all-zero tile).
+template <int NT = kNT>
 __device__ __forceinline__ int tile_shift(uint32_t mb, uint32_t *red, uint32_t tid, int target) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mb = max(mb, (uint32_t)__shfl_xor((int)mb, off));
@@ -349,7 +350,7 @@ __device__ __forceinline__ int tile_shift(uint32_t mb, uint32_t *red, uint32_t t
     __syncthreads();
     uint32_t m = red[0];
 #pragma unroll
-    for (int w = 1; w < kNT / 64; ++w) m = max(m, red[w]);
+    for (int w = 1; w < NT / 64; ++w) m = max(m, red[w]);
     if (m == 0) return 0;
     const int efield = (int)(m >> 23);
     const int ex = efield == 0 ? -125 : efield - 126;   // largest component < 2^ex
@@ -360,7 +361,7 @@ __device__ __forceinline__ int tile_shift(uint32_t mb, uint32_t *red, uint32_t t
 // 2^shift so that its largest component is below 2^target; returns the shift.  With K <= 13 and
 // at least one contiguous low bit every element of the thread is loaded into registers first
 // (all loads in flight at once), otherwise the tile is loaded and then rescaled in LDS.
-template <int K>
+template <int K, int NT = kNT>
 __device__ __forceinline__ int load_tile_scaled(C2<float> *__restrict__ st, C2<float> *tile, const uint64_t *sbo,
                                                 const uint64_t *tb, int k, int lowb, uint32_t tid, uint32_t *red,
                                                 int target) {
@@ -369,11 +370,11 @@ __device__ __forceinline__ int load_tile_scaled(C2<float> *__restrict__ st, C2<f
     uint32_t mb = 0;
     if constexpr (K <= 13) {
         if (lowb >= 1) {
-            constexpr int IT = NE / (2 * kNT);
+            constexpr int IT = NE / (2 * NT);
             float4 v[IT];
 #pragma unroll
             for (int i = 0; i < IT; ++i) {
-                const uint32_t e = 2u * (i * kNT + tid);
+                const uint32_t e = 2u * (i * NT + tid);
                 const uint32_t j = e & tmask, u = e >> k;
                 const uint64_t g = tb[u] | sbo[j >> lowb] | (j & lowmask);
                 v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * g);
@@ -381,25 +382,25 @@ __device__ __forceinline__ int load_tile_scaled(C2<float> *__restrict__ st, C2<f
 #pragma unroll
             for (int i = 0; i < IT; ++i)
                 mb = max(mb, max(max(absbits(v[i].x), absbits(v[i].y)), max(absbits(v[i].z), absbits(v[i].w))));
-            const int sh = tile_shift(mb, red, tid, target);
+            const int sh = tile_shift<NT>(mb, red, tid, target);
 #pragma unroll
             for (int i = 0; i < IT; ++i) {
-                const uint32_t pe = lpad(2u * (i * kNT + tid));
+                const uint32_t pe = lpad(2u * (i * NT + tid));
                 tile[pe] = C2<float>{ldexpf(v[i].x, sh), ldexpf(v[i].y, sh)};
                 tile[pe + 1] = C2<float>{ldexpf(v[i].z, sh), ldexpf(v[i].w, sh)};
             }
             return sh;
         }
     }
-    move_tile<float, K, true>(st, tile, sbo, tb, k, lowb, tid);
+    move_tile<float, K, true, NT>(st, tile, sbo, tb, k, lowb, tid);
     __syncthreads();
-    for (uint32_t e = tid; e < NE; e += kNT) {
+    for (uint32_t e = tid; e < NE; e += NT) {
         const C2<float> x = tile[lpad(e)];
         mb = max(mb, max(absbits(x.x), absbits(x.y)));
     }
-    const int sh = tile_shift(mb, red, tid, target);
+    const int sh = tile_shift<NT>(mb, red, tid, target);
     if (sh != 0)
-        for (uint32_t e = tid; e < NE; e += kNT) {
+        for (uint32_t e = tid; e < NE; e += NT) {
             const C2<float> x = tile[lpad(e)];
             tile[lpad(e)] = C2<float>{ldexpf(x.x, sh), ldexpf(x.y, sh)};
         }
@@ -637,8 +638,19 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
     }
 }
 
+// 4-qubit dense group only (the dense-only instance's register budget)
+template <int K, int NWAVE>
+__device__ __forceinline__ void dense_group4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                             uint32_t wt) {
+    if (G.type == 1) dense_apply<K, 4, NWAVE, false>(buf, G, wd, wt);
+    else if (G.type == 4) dense_apply_h<K, 4, NWAVE, false, true>(buf, G, wd, wt);
+    else dense_apply_h<K, 4, NWAVE, false, false>(buf, G, wd, wt);
+}
+
 // One workgroup tile of one pass: `tile` selects the 2^(K-k) QSV2 tiles the workgroup owns.
-template <typename T, int K, bool DN>
+// DN: the pass has dense groups; D4: dense-only instance (every group a dense 4-qubit unit, NT threads,
+// no MUX2 register groups compiled in).
+template <typename T, int K, bool DN, bool D4 = false, int NT = kNT>
 __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                           const GroupDev *__restrict__ gd, const OpDev<T> *__restrict__ od,
                                           const float *__restrict__ wd, uint64_t tile_id, unsigned char *smem) {
@@ -651,13 +663,13 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
     uint64_t *tb = sbo + (1u << m);
     const uint32_t tid = threadIdx.x;
 
-    for (uint32_t s = tid; s < (1u << m); s += kNT) {
+    for (uint32_t s = tid; s < (1u << m); s += NT) {
         uint64_t o = 0;
         for (int i = 0; i < m; ++i)
             if ((s >> i) & 1u) o |= uint64_t(1) << pd->h[i];
         sbo[s] = o;
     }
-    for (uint32_t u = tid; u < (1u << sub); u += kNT) {
+    for (uint32_t u = tid; u < (1u << sub); u += NT) {
         uint64_t x = ((tile_id << sub) | u) << (k - m);
         for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
         tb[u] = x;
@@ -667,23 +679,28 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
     int tsh = 0;   // tile scaling shift (passes with pre-scaled dense groups)
     if constexpr (DN && sizeof(T) == 4) {
         if (pd->tscale > 0)
-            tsh = load_tile_scaled<K>(reinterpret_cast<C2<float> *>(st), reinterpret_cast<C2<float> *>(tile), sbo, tb,
-                                      k, lowb, tid, reinterpret_cast<uint32_t *>(tb + (1u << sub)), pd->tscale);
+            tsh = load_tile_scaled<K, NT>(reinterpret_cast<C2<float> *>(st), reinterpret_cast<C2<float> *>(tile), sbo,
+                                          tb, k, lowb, tid, reinterpret_cast<uint32_t *>(tb + (1u << sub)), pd->tscale);
         else
-            move_tile<T, K, true>(st, tile, sbo, tb, k, lowb, tid);
+            move_tile<T, K, true, NT>(st, tile, sbo, tb, k, lowb, tid);
     } else {
-        move_tile<T, K, true>(st, tile, sbo, tb, k, lowb, tid);
+        move_tile<T, K, true, NT>(st, tile, sbo, tb, k, lowb, tid);
     }
     __syncthreads();
 
-    constexpr int NR = 1 << kR;
-    constexpr uint32_t NSLOT = NE >> kR;
+    constexpr int RR = kR;   // register bits of a MUX2 group
+    constexpr int NR = 1 << RR;
+    constexpr uint32_t NSLOT = NE >> RR;
     const int ng = pd->ngroups, g0 = pd->group0;
     for (int g = 0; g < ng; ++g) {
         const GroupDev &G = gd[g0 + g];
-        if constexpr (DN && sizeof(T) == 4) {
+        if constexpr (D4 && sizeof(T) == 4) {
+            dense_group4<K, NT / 64>(smem, G, wd, tid);   // dense-only passes: no MUX2 path compiled in
+            __syncthreads();
+            continue;
+        } else if constexpr (DN && sizeof(T) == 4) {
             if (G.type != 0) {
-                dense_group<K, kNT / 64, false>(smem, G, wd, tid);
+                dense_group<K, NT / 64, false>(smem, G, wd, tid);
                 __syncthreads();
                 continue;
             }
@@ -691,17 +708,17 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
         uint32_t poff[NR];   // padded LDS offset of register r (disjoint bits: pad(a|b) = pad(a)+pad(b))
         poff[0] = 0;
 #pragma unroll
-        for (int q = 0; q < kR; ++q) {
+        for (int q = 0; q < RR; ++q) {
             const uint32_t bq = lpad(1u << G.b[q]);
 #pragma unroll
             for (int r = (1 << q); r < (2 << q); ++r) poff[r] = poff[r - (1 << q)] + bq;
         }
         const int nops = G.nops, o0 = G.op0;
 #pragma unroll 1
-        for (uint32_t s = tid; s < NSLOT; s += kNT) {
+        for (uint32_t s = tid; s < NSLOT; s += NT) {
             uint32_t base = s;
 #pragma unroll
-            for (int q = 0; q < kR; ++q) base = ins0_32(base, G.bs[q]);
+            for (int q = 0; q < RR; ++q) base = ins0_32(base, G.bs[q]);
             const uint32_t pb = lpad(base);
             C2<T> a[NR];
 #pragma unroll
@@ -710,14 +727,14 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
             for (int o = 0; o < nops; ++o) {
                 const OpDev<T> &op = od[o0 + o];
                 const bool ts = (base & op.cmt) == op.cmt;
-                apply_op<T, kR>(a, op, ts);
+                apply_op<T, RR>(a, op, ts);
             }
 #pragma unroll
             for (int r = 0; r < NR; ++r) tile[pb + poff[r]] = a[r];
         }
         __syncthreads();
     }
-    move_tile<T, K, false>(st, tile, sbo, tb, k, lowb, tid, -tsh);
+    move_tile<T, K, false, NT>(st, tile, sbo, tb, k, lowb, tid, -tsh);
 }
 
 template <typename T, int K, bool DN>
@@ -726,6 +743,17 @@ __global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass(C2<T> *__r
                                               const OpDev<T> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
     pass_tile<T, K, DN>(st, pd, gd, od, wd, blockIdx.x, smem);
+}
+
+// Dense-only passes with 4-qubit units: 512 threads and at most 128 VGPRs (90 used), so two workgroups
+// give four waves per SIMD to hide one wave's LDS -> split -> MFMA -> LDS chain behind the others.
+constexpr int kNTD4 = 512;
+template <int K>
+__global__ __launch_bounds__(kNTD4, 4) void k_pass_d4(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
+                                                      const GroupDev *__restrict__ gd,
+                                                      const OpDev<float> *__restrict__ od, const float *__restrict__ wd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pass_tile<float, K, true, true, kNTD4>(st, pd, gd, od, wd, blockIdx.x, smem);
 }
 
 // Continuous batching: one launch runs one pass of each of several states (the items, one kernel
@@ -1138,8 +1166,43 @@ hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const
                            const OpDev<float> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s) {
     return launch_pass_t<float>(K, small, st, pd, gd, od, wd, grid, lds, s);
 }
-hipError_t launch_pass_batch(int K, bool dense, const BatchItem *items, int n_items, uint32_t grid, size_t lds,
-                             hipStream_t s) {
+hipError_t launch_pass_d4(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+                          const float *wd, uint32_t grid, size_t lds, hipStream_t s) {
+    C2<float> *p = reinterpret_cast<C2<float> *>(st);
+    switch (K) {
+    case 12: hipLaunchKernelGGL((k_pass_d4<12>), dim3(grid), dim3(kNTD4), lds, s, p, pd, gd, od, wd); break;
+    case 13: hipLaunchKernelGGL((k_pass_d4<13>), dim3(grid), dim3(kNTD4), lds, s, p, pd, gd, od, wd); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// batched launches of dense-only passes of 4-qubit units (the 512-thread kernel's passes)
+template <int K>
+__global__ __launch_bounds__(kNTD4, 4) void k_pass_batch_d4(const BatchItem *__restrict__ items, int n_items) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t b = blockIdx.x;
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchItem &it = items[lo];
+    pass_tile<float, K, true, true, kNTD4>(reinterpret_cast<C2<float> *>(it.st), it.pd, it.gd,
+                                           reinterpret_cast<const OpDev<float> *>(it.od), it.wd, b - it.tile0, smem);
+}
+
+hipError_t launch_pass_batch(int K, bool dense, bool d4, const BatchItem *items, int n_items, uint32_t grid,
+                             size_t lds, hipStream_t s) {
+    if (d4) {
+        switch (K) {
+        case 12: hipLaunchKernelGGL((k_pass_batch_d4<12>), dim3(grid), dim3(kNTD4), lds, s, items, n_items); break;
+        case 13: hipLaunchKernelGGL((k_pass_batch_d4<13>), dim3(grid), dim3(kNTD4), lds, s, items, n_items); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (K * 2 + (dense ? 1 : 0)) {
     case 20: hipLaunchKernelGGL((k_pass_batch<10, false>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
     case 21: hipLaunchKernelGGL((k_pass_batch<10, true>), dim3(grid), dim3(kNT), lds, s, items, n_items); break;
@@ -1482,6 +1545,7 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<float, 13, false>)) SVX_ATTR((k_pass<float, 14, false>))
     SVX_ATTR((k_pass<float, 10, true>)) SVX_ATTR((k_pass<float, 11, true>)) SVX_ATTR((k_pass<float, 12, true>))
     SVX_ATTR((k_pass<float, 13, true>)) SVX_ATTR((k_pass<float, 14, true>))
+    SVX_ATTR((k_pass_d4<12>)) SVX_ATTR((k_pass_d4<13>)) SVX_ATTR((k_pass_batch_d4<12>)) SVX_ATTR((k_pass_batch_d4<13>))
     SVX_ATTR((k_pass_batch<10, false>)) SVX_ATTR((k_pass_batch<10, true>)) SVX_ATTR((k_pass_batch<11, false>))
     SVX_ATTR((k_pass_batch<11, true>)) SVX_ATTR((k_pass_batch<12, false>)) SVX_ATTR((k_pass_batch<12, true>))
     SVX_ATTR((k_pass_batch<13, false>)) SVX_ATTR((k_pass_batch<13, true>)) SVX_ATTR((k_pass_batch<14, false>))
