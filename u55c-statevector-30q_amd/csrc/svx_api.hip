@@ -470,6 +470,8 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     }
     const char *d4env = std::getenv("SVX_D4");
     const std::string d4_mode = d4env ? d4env : "auto";
+    const char *oenv = std::getenv("SVX_TILE_ORDER");
+    const int torder = oenv ? std::atoi(oenv) : -2;   // default: each XCD streams a contiguous range of tiles
     if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
         double c4 = fcfg.cost4, c5 = fcfg.cost5;
         int wd = fcfg.width;
@@ -575,6 +577,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         pd.m = rp.m;
         pd.K = K;
         pd.nbits = nbits;
+        pd.torder = torder;
         for (int i = 0; i < rp.m; ++i) pd.h[i] = pd.hs[i] = rp.h[i];
         std::sort(pd.hs, pd.hs + rp.m);
         if (mode >= 2) {
@@ -836,9 +839,10 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     const char *cenv = std::getenv("SVX_FUSE_COST");
     const char *tenv = std::getenv("SVX_TILE_SCALE");
     const char *d4env = std::getenv("SVX_D4");
+    const char *oenv = std::getenv("SVX_TILE_ORDER");
     const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
                              (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "") + "/" +
-                             (d4env ? d4env : "");
+                             (d4env ? d4env : "") + "/" + (oenv ? oenv : "");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);

@@ -36,7 +36,8 @@ struct PassDev {
     int32_t hs[kMaxM];           // h sorted ascending (tile-id deposit)
     int32_t tscale;              // > 0: scale each LDS tile so its largest component is < 2^tscale
                                  // (dense groups of type 4 rely on it); 0: no tile scaling
-    int32_t pad0;
+    int32_t torder;              // workgroup -> tile order: 0 identity, -1 bit reversal, -2 XCD-contiguous,
+                                 // r > 0 rotate left by r
     uint64_t dma_u[64];          // DMA kernel: element offset of tile elements 128 j .. from bits >= 7
 };
 

@@ -737,12 +737,28 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
     move_tile<T, K, false, NT>(st, tile, sbo, tb, k, lowb, tid, -tsh);
 }
 
+// Order in which the workgroups of a pass take the 2^tbits tiles (tbits = nbits - K).  Workgroups are
+// dispatched round-robin over the 8 XCDs; the default (-2) gives each XCD one contiguous eighth of the
+// tiles, which measured 3-18% faster than the identity order on light 30-qubit passes.
+__device__ __forceinline__ uint64_t tile_order(uint64_t b, const PassDev *pd) {
+    const int tbits = pd->nbits - pd->K, r = pd->torder;
+    if (r == 0 || tbits <= 1) return b;
+    if (r == -2) {   // XCD-contiguous: workgroups go round-robin over 8 XCDs; XCD x takes tiles x T/8 ..
+        if (tbits < 3) return b;
+        return ((b & 7u) << (tbits - 3)) | (b >> 3);
+    }
+    if (r < 0) return __builtin_bitreverse64(b) >> (64 - tbits);
+    const int rr = r % tbits;
+    const uint64_t mask = (uint64_t(1) << tbits) - 1;
+    return ((b << rr) | (b >> (tbits - rr))) & mask;
+}
+
 template <typename T, int K, bool DN>
 __global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                               const GroupDev *__restrict__ gd,
                                               const OpDev<T> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    pass_tile<T, K, DN>(st, pd, gd, od, wd, blockIdx.x, smem);
+    pass_tile<T, K, DN>(st, pd, gd, od, wd, tile_order(blockIdx.x, pd), smem);
 }
 
 // Dense-only passes with 4-qubit units: 512 threads and at most 128 VGPRs (90 used), so two workgroups
@@ -753,7 +769,7 @@ __global__ __launch_bounds__(kNTD4, 4) void k_pass_d4(C2<float> *__restrict__ st
                                                       const GroupDev *__restrict__ gd,
                                                       const OpDev<float> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    pass_tile<float, K, true, true, kNTD4>(st, pd, gd, od, wd, blockIdx.x, smem);
+    pass_tile<float, K, true, true, kNTD4>(st, pd, gd, od, wd, tile_order(blockIdx.x, pd), smem);
 }
 
 // Continuous batching: one launch runs one pass of each of several states (the items, one kernel
@@ -772,7 +788,7 @@ __global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass_batch(cons
     }
     const BatchItem &it = items[lo];
     pass_tile<float, K, DN>(reinterpret_cast<C2<float> *>(it.st), it.pd, it.gd,
-                            reinterpret_cast<const OpDev<float> *>(it.od), it.wd, b - it.tile0, smem);
+                            reinterpret_cast<const OpDev<float> *>(it.od), it.wd, tile_order(b - it.tile0, it.pd), smem);
 }
 
 // ------------------------------------------------------------------------------------------ pass, DMA
@@ -1190,7 +1206,8 @@ __global__ __launch_bounds__(kNTD4, 4) void k_pass_batch_d4(const BatchItem *__r
     }
     const BatchItem &it = items[lo];
     pass_tile<float, K, true, true, kNTD4>(reinterpret_cast<C2<float> *>(it.st), it.pd, it.gd,
-                                           reinterpret_cast<const OpDev<float> *>(it.od), it.wd, b - it.tile0, smem);
+                                           reinterpret_cast<const OpDev<float> *>(it.od), it.wd,
+                                           tile_order(b - it.tile0, it.pd), smem);
 }
 
 hipError_t launch_pass_batch(int K, bool dense, bool d4, const BatchItem *items, int n_items, uint32_t grid,
