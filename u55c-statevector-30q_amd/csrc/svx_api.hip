@@ -240,6 +240,7 @@ struct LaunchInfo {
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
+    bool d4p;             // ... in its persistent form (next tile's loads in flight)
     int32_t stats[5];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32)
 };
 
@@ -687,8 +688,10 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 } else {
                     // split-FP16 A operands of v_mfma_f32_32x32x16_f16: 2^wexp W = Wh + Wl with the
                     // largest |entry| of 2^wexp W below 2^14; block ((rt NKS + ks) 2 + part) holds
-                    // lane l, entry e = W[32 rt + l % 32][16 ks + 8 (l / 32) + e]
+                    // lane l, entry e = Wi[32 rt + l % 32][16 ks + 8 (l / 32) + e], Wi the interleaved
+                    // real form (index 2c + p: component c, part p) -- Wi[r][s] = W[il(r)][il(s)]
                     gd.type = 3;
+                    auto il = [D](int x) { return (x & 1) * D + (x >> 1); };
                     const int NKS = D / 8;
                     double wmax = 0;
                     for (double v : du.u) wmax = std::max(wmax, std::fabs(v));
@@ -705,7 +708,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                             for (int part = 0; part < 2; ++part)
                                 for (int l = 0; l < 64; ++l)
                                     for (int e = 0; e < 8; ++e) {
-                                        const double ws = W(32 * rt + l % 32, 16 * ks + 8 * (l / 32) + e) * wsc;
+                                        const double ws = W(il(32 * rt + l % 32), il(16 * ks + 8 * (l / 32) + e)) * wsc;
                                         const uint16_t hh = f32_to_f16_bits((float)ws);
                                         hb.push_back(part == 0 ? hh : f32_to_f16_bits((float)(ws - f16_bits_to_f32(hh))));
                                     }
@@ -790,10 +793,16 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         pd.ngroups = (int)groups.size() - pd.group0;
         LaunchInfo &lr = launches.back();
         // dense-only passes of 4-qubit units run on the 512-thread kernel (SVX_D4=off: never)
-        if (mode == 1 && has_dense && d4_mode != "off" && (K == 12 || K == 13) && pd.ngroups > 0) {
+        if (mode == 1 && has_dense && d4_mode != "off" && (K == 12 || K == 13) && pd.ngroups > 0 && k - rp.m >= 1) {
             bool all4 = true;
             for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].f == 4;
             lr.d4 = all4;
+            // persistent form: one QSV2 tile per workgroup tile with a contiguous low bit, two
+            // workgroups per CU (SVX_D4=pers)
+            if (all4 && d4_mode == "pers" && K == k && k - rp.m >= 1) {
+                lr.d4p = true;
+                lr.grid = (uint32_t)std::min<uint64_t>(lr.ntiles, 2 * (uint64_t)ds->caps.cu_count);
+            }
         }
         for (int g = pd.group0; g < (int)groups.size(); ++g) {
             const GroupDev &G = groups[g];
@@ -886,7 +895,10 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
     for (const LaunchInfo &li : pr->launches) {
         if (li.empty) continue;
         hipError_t e;
-        if (li.d4)
+        if (li.d4p)
+            e = launch_pass_d4p(li.K, state, pd + li.pass_index, gd,
+                                reinterpret_cast<const float *>(base + pr->w_off), li.grid, li.lds, s);
+        else if (li.d4)
             e = launch_pass_d4(li.K, state, pd + li.pass_index, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
                                li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr, li.grid, li.lds, s);
         else if (li.pers)
@@ -1120,6 +1132,7 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
     else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
     else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5" : ",4") + (li.dense ? ",dense>" : ">");
+    else if (li.d4p) k = "k_pass_d4p<" + std::to_string(li.K) + ">";
     else if (li.d4) k = "k_pass_d4<" + std::to_string(li.K) + ">";
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());

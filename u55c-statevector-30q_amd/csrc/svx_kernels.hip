@@ -512,18 +512,36 @@ __device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &
 // by a power of two s (per tile or per column, max |x s| < 2^15) and split x s = xh + xl in FP16;
 // y = (Wh xh + Wh xl + Wl xh) / (2^wexp s) with FP32 accumulation.  The dropped product and the
 // two-piece representations leave a relative error of about 2^-21 of the column's largest entry
-// per product.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B)
-// l % 32 and K entries 8 (l / 32) + 0..7 of each 16-deep K step.
+// per product.  v_mfma_f32_32x32x16_f16: lane l holds row (A) / column (B) l % 32 and K entries
+// 8 (l / 32) + 0..7 of each 16-deep K step.
+//
+// The real form is interleaved: real index 2c + p is component c, part p (0 re, 1 im), for rows and
+// columns alike, so W[2a + p][2c + q] = (Ur, -Ui; Ui, Ur)[p][q] (a, c).  K step ks then holds
+// components 8 ks .. 8 ks + 7, lane half h the four components 8 ks + 4 h + 0..3 as (re, im) pairs:
+// one complex amplitude converts to one packed FP16 pair.  Accumulators v, v + 1 (v even) of 32-row
+// tile rt are re, im of component 16 rt + 4 (v / 4) + 2 h + (v % 4) / 2 and store as one 8-byte pair.
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// (re, im) = hi + lo in FP16 pairs (re, im both below 2^15 in magnitude): hi keeps the top 11
+// significant bits (a mask, then an exact conversion; the FP16 subnormal grid below 2^-14), lo =
+// (re, im) - hi -- exact in FP32 -- rounded to the nearest FP16, so the split does not shrink
+// amplitudes on average and the norm does not drift over many dense ops.
+__device__ __forceinline__ void split_pk(float re, float im, uint32_t &hi, uint32_t &lo) {
+    const float rh = __uint_as_float(__float_as_uint(re) & 0xffffe000u);
+    const float ih = __uint_as_float(__float_as_uint(im) & 0xffffe000u);
+    hi = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(rh, ih));
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(float2v{re - rh, im - ih}, half2v));
+}
 
 template <int K, int F, int NWAVE, bool SWZ, bool PRE>
 __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                               uint32_t wt) {
     constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
     constexpr int D = 1 << F;
-    constexpr int NG = D / 16;                     // groups of 8 components per lane half
-    constexpr int NKS = 2 * NG;                    // 16-deep K steps (real parts, then imaginary parts)
+    constexpr int NKS = D / 8;                     // 16-deep K steps of 8 interleaved components
     constexpr uint32_t NB = (1u << (K - F)) / 32;
     const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
     const half8 *w8 = reinterpret_cast<const half8 *>(wd + G.w_off);
@@ -537,86 +555,104 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
         }
     ColOff<F, SWZ> coff;
     coff.init(G);
-    for (uint32_t b = wave; b < NB; b += NWAVE) {
-        uint32_t base = b * 32u + j;
+    // column b 32 + j sits at tile index dep(b 32) | dep(j) (dep: spread the bits over the positions
+    // outside the unit's qubits; disjoint source bits spread to disjoint positions, and both tile
+    // layouts are additive / XOR-linear over disjoint bits): the lane part once, the wave-uniform
+    // part in scalar registers per step
+    uint32_t dj = j;
 #pragma unroll
-        for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
-        const uint32_t pb = TileAddr<SWZ>::of(base);
-        C2<float> x[NG][8];
+    for (int i = 0; i < F; ++i) dj = ins0_32(dj, G.dqs[i]);
+    const uint32_t pj = TileAddr<SWZ>::of(dj);
+    // NP independent 32-column batches per step (b and b + NWAVE): one batch's MFMA chain overlaps
+    // the other's LDS reads, split and stores within the wavefront (4-qubit units: the registers fit)
+    constexpr int NP = (F == 4 && NB % (2u * NWAVE) == 0) ? 2 : 1;
+    for (uint32_t b = __builtin_amdgcn_readfirstlane(wave); b < NB; b += NP * NWAVE) {
+        // the lane half's component bit goes into the lane base, the rest of each component's
+        // offset is uniform (scalar registers)
+        uint32_t pbl[NP], pbs[NP];
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
+        for (int p = 0; p < NP; ++p) {
+            uint32_t db = (b + p * NWAVE) * 32u;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) x[g][e] = ldc(buf, TileAddr<SWZ>::comb(pb, coff(16u * g + 8u * h + e)));
-        int sh = 0;   // pre-scaled tile: every component is already below 2^15
-        if constexpr (!PRE) {
-            uint32_t mb = 0;
-#pragma unroll
-            for (int g = 0; g < NG; ++g)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) mb = max(mb, max(absbits(x[g][e].x), absbits(x[g][e].y)));
-            mb = max(mb, (uint32_t)__shfl_xor((int)mb, 32));
-            const int efield = (int)(mb >> 23);
-            sh = mb == 0 ? 0 : 15 - (efield == 0 ? -125 : efield - 126);   // column maximum < 2^15
+            for (int i = 0; i < F; ++i) db = ins0_32(db, G.dqs[i]);
+            const uint32_t pb = TileAddr<SWZ>::comb(pj, TileAddr<SWZ>::of(db));
+            pbl[p] = TileAddr<SWZ>::comb(pb, h ? coff.lq[2] : 0u);
+            pbs[p] = TileAddr<SWZ>::comb(pb, h ? coff.lq[1] : 0u);
         }
-        // x s = xh + xl: xh = x s truncated to 11 significant bits (a mask; an exact FP16 value down
-        // to 2^-14, i.e. 2^-28 of the column maximum), xl = x s - xh (exact in FP32) rounded to the
-        // nearest FP16, so the split does not shrink amplitudes on average and the norm does not
-        // drift over many dense ops.
-        half8 bh[NKS], bl[NKS];
+        C2<float> x[NP][NKS][4];
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
-#pragma unroll
-            for (int e = 0; e < 8; e += 2) {
-                float xs[2][2], xh[2][2];
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    xs[0][u] = PRE ? x[g][e + u].x : ldexpf(x[g][e + u].x, sh);
-                    xs[1][u] = PRE ? x[g][e + u].y : ldexpf(x[g][e + u].y, sh);
-#pragma unroll
-                    for (int p = 0; p < 2; ++p) xh[p][u] = __uint_as_float(__float_as_uint(xs[p][u]) & 0xffffe000u);
-                }
-#pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    const auto hp = __builtin_amdgcn_cvt_pkrtz(xh[p][0], xh[p][1]);
-                    const half2v lp = __builtin_convertvector(float2v{xs[p][0] - xh[p][0], xs[p][1] - xh[p][1]}, half2v);
-                    bh[p * NG + g][e] = hp[0];
-                    bh[p * NG + g][e + 1] = hp[1];
-                    bl[p * NG + g][e] = lp[0];
-                    bl[p * NG + g][e + 1] = lp[1];
-                }
-            }
-        f16v acc[NRT];
-#pragma unroll
-        for (int rt = 0; rt < NRT; ++rt) {
-            acc[rt] = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int ks = 0; ks < NKS; ++ks) {
-                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rt][ks], bh[ks], acc[rt], 0, 0, 0);
-                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bl[ks], acc[rt], 0, 0, 0);
-            }
+        for (int p = 0; p < NP; ++p)
 #pragma unroll
             for (int ks = 0; ks < NKS; ++ks)
-                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bh[ks], acc[rt], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) x[p][ks][i] = ldc(buf, TileAddr<SWZ>::comb(pbl[p], coff(8u * ks + i)));
+        int sh[NP];   // pre-scaled tile: every component is already below 2^15
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            sh[p] = 0;
+            if constexpr (!PRE) {
+                uint32_t mb = 0;
+#pragma unroll
+                for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) mb = max(mb, max(absbits(x[p][ks][i].x), absbits(x[p][ks][i].y)));
+                mb = max(mb, (uint32_t)__shfl_xor((int)mb, 32));
+                const int efield = (int)(mb >> 23);
+                sh[p] = mb == 0 ? 0 : 15 - (efield == 0 ? -125 : efield - 126);   // column maximum < 2^15
+            }
         }
-        const int back = -(sh + G.wexp);
-        if (back != 0) {
+        half8 bh[NP][NKS], bl[NP][NKS];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                u4v hv, lv;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float re = PRE ? x[p][ks][i].x : ldexpf(x[p][ks][i].x, sh[p]);
+                    const float im = PRE ? x[p][ks][i].y : ldexpf(x[p][ks][i].y, sh[p]);
+                    uint32_t hh, ll;
+                    split_pk(re, im, hh, ll);
+                    hv[i] = hh;
+                    lv[i] = ll;
+                }
+                bh[p][ks] = __builtin_bit_cast(half8, hv);
+                bl[p][ks] = __builtin_bit_cast(half8, lv);
+            }
+        f16v acc[NP][NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) acc[p][rt] = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    acc[p][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rt][ks], bh[p][ks], acc[p][rt], 0, 0, 0);
+                    acc[p][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bl[p][ks], acc[p][rt], 0, 0, 0);
+                }
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    acc[p][rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rt][ks], bh[p][ks], acc[p][rt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int back = -(sh[p] + G.wexp);
+            if (back != 0) {
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) acc[p][rt][v] = ldexpf(acc[p][rt][v], back);
+            }
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-                for (int v = 0; v < 16; ++v) acc[rt][v] = ldexpf(acc[rt][v], back);
-        }
-        if (F == 4) {
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[0][v + 8]});
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[NRT - 1][v]});
-            }
+                for (int v = 0; v < 16; v += 2) {
+                    const uint32_t c = 16u * rt + 4u * (v / 4) + (v % 4) / 2;   // + 2 h: in pbs
+                    stc(buf, TileAddr<SWZ>::comb(pbs[p], coff(c)), C2<float>{acc[p][rt][v], acc[p][rt][v + 1]});
+                }
         }
     }
 }
@@ -639,12 +675,70 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
 }
 
 // 4-qubit dense group only (the dense-only instance's register budget)
-template <int K, int NWAVE>
+template <int K, int NWAVE, bool SWZ = false>
 __device__ __forceinline__ void dense_group4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                              uint32_t wt) {
-    if (G.type == 1) dense_apply<K, 4, NWAVE, false>(buf, G, wd, wt);
-    else if (G.type == 4) dense_apply_h<K, 4, NWAVE, false, true>(buf, G, wd, wt);
-    else dense_apply_h<K, 4, NWAVE, false, false>(buf, G, wd, wt);
+    if (G.type == 1) dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
+    else if (G.type == 4) dense_apply_h<K, 4, NWAVE, SWZ, true>(buf, G, wd, wt);
+    else dense_apply_h<K, 4, NWAVE, SWZ, false>(buf, G, wd, wt);
+}
+
+// Dense-only passes keep the tile XOR-swizzled in LDS (swz: amplitudes 2i, 2i + 1 share one 16-byte
+// slot, slot i ^ ((2i >> 5) & 15)), moved with 16-byte LDS accesses; every QSV2 tile has at least
+// one contiguous low bit (k - m >= 1), so each thread's 16-byte HBM access is one slot.
+// Load: returns the tile scaling shift (target > 0: largest component below 2^target).
+template <int K, int NT>
+__device__ __forceinline__ int tile_in_swz(const C2<float> *__restrict__ st, unsigned char *buf, const uint64_t *sbo,
+                                           const uint64_t *tb, int k, int lowb, uint32_t tid, uint32_t *red,
+                                           int target) {
+    constexpr int IT = (1 << K) / (2 * NT);
+    const uint32_t lowmask = (1u << lowb) - 1u, tmask = (1u << k) - 1u;
+    float4 v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const uint32_t e = 2u * (i * NT + tid);
+        const uint32_t j = e & tmask, u = e >> k;
+        v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * (tb[u] | sbo[j >> lowb] | (j & lowmask)));
+    }
+    int sh = 0;
+    if (target > 0) {
+        uint32_t mb = 0;
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            mb = max(mb, max(max(absbits(v[i].x), absbits(v[i].y)), max(absbits(v[i].z), absbits(v[i].w))));
+        sh = tile_shift<NT>(mb, red, tid, target);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        float4 w = v[i];
+        if (sh) w = make_float4(ldexpf(w.x, sh), ldexpf(w.y, sh), ldexpf(w.z, sh), ldexpf(w.w, sh));
+        *reinterpret_cast<float4 *>(buf + swz(2u * (i * NT + tid))) = w;
+    }
+    return sh;
+}
+template <int K, int NT>
+__device__ __forceinline__ void tile_out_swz(C2<float> *__restrict__ st, const unsigned char *buf, const uint64_t *sbo,
+                                             const uint64_t *tb, int k, int lowb, uint32_t tid, int unscale) {
+    constexpr int IT = (1 << K) / (2 * NT);
+    constexpr int BATCH = IT < 4 ? IT : 4;   // registers: four 16-byte values in flight per step
+    const uint32_t lowmask = (1u << lowb) - 1u, tmask = (1u << k) - 1u;
+#pragma unroll
+    for (int i0 = 0; i0 < IT; i0 += BATCH) {
+        float4 w[BATCH];
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) {
+            w[i] = *reinterpret_cast<const float4 *>(buf + swz(2u * ((i0 + i) * NT + tid)));
+            if (unscale)
+                w[i] = make_float4(ldexpf(w[i].x, unscale), ldexpf(w[i].y, unscale), ldexpf(w[i].z, unscale),
+                                   ldexpf(w[i].w, unscale));
+        }
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) {
+            const uint32_t e = 2u * ((i0 + i) * NT + tid);
+            const uint32_t j = e & tmask, u = e >> k;
+            st_stream(reinterpret_cast<float *>(st) + 2 * (tb[u] | sbo[j >> lowb] | (j & lowmask)), w[i]);
+        }
+    }
 }
 
 // One workgroup tile of one pass: `tile` selects the 2^(K-k) QSV2 tiles the workgroup owns.
@@ -677,7 +771,10 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
     __syncthreads();
     const int lowb = k - m;
     int tsh = 0;   // tile scaling shift (passes with pre-scaled dense groups)
-    if constexpr (DN && sizeof(T) == 4) {
+    if constexpr (D4 && sizeof(T) == 4) {
+        tsh = tile_in_swz<K, NT>(reinterpret_cast<const C2<float> *>(st), smem, sbo, tb, k, lowb, tid,
+                                 reinterpret_cast<uint32_t *>(tb + (1u << sub)), pd->tscale);
+    } else if constexpr (DN && sizeof(T) == 4) {
         if (pd->tscale > 0)
             tsh = load_tile_scaled<K, NT>(reinterpret_cast<C2<float> *>(st), reinterpret_cast<C2<float> *>(tile), sbo,
                                           tb, k, lowb, tid, reinterpret_cast<uint32_t *>(tb + (1u << sub)), pd->tscale);
@@ -695,7 +792,7 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
     for (int g = 0; g < ng; ++g) {
         const GroupDev &G = gd[g0 + g];
         if constexpr (D4 && sizeof(T) == 4) {
-            dense_group4<K, NT / 64>(smem, G, wd, tid);   // dense-only passes: no MUX2 path compiled in
+            dense_group4<K, NT / 64, true>(smem, G, wd, tid);   // dense-only passes: no MUX2 path compiled in
             __syncthreads();
             continue;
         } else if constexpr (DN && sizeof(T) == 4) {
@@ -723,18 +820,29 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
             C2<T> a[NR];
 #pragma unroll
             for (int r = 0; r < NR; ++r) a[r] = tile[pb + poff[r]];
+            // the next op's descriptor (scalar loads) is in flight while this one is applied.
+            // Scalar loads return out of order, so the wait for the current descriptor has to
+            // come before the next one is issued: the empty asm reads it, the scheduling
+            // barrier keeps the next load behind that read.
+            OpDev<T> cur = od[o0];
 #pragma unroll 1
             for (int o = 0; o < nops; ++o) {
-                const OpDev<T> &op = od[o0 + o];
-                const bool ts = (base & op.cmt) == op.cmt;
-                apply_op<T, RR>(a, op, ts);
+                asm volatile("" ::"s"(cur.kind), "s"(cur.tr), "s"(cur.cmt), "s"(cur.cmr));
+                __builtin_amdgcn_sched_barrier(0);
+                const OpDev<T> nxt = od[o0 + min(o + 1, nops - 1)];
+                const bool ts = (base & cur.cmt) == cur.cmt;
+                apply_op<T, RR>(a, cur, ts);
+                cur = nxt;
             }
 #pragma unroll
             for (int r = 0; r < NR; ++r) tile[pb + poff[r]] = a[r];
         }
         __syncthreads();
     }
-    move_tile<T, K, false, NT>(st, tile, sbo, tb, k, lowb, tid, -tsh);
+    if constexpr (D4 && sizeof(T) == 4)
+        tile_out_swz<K, NT>(reinterpret_cast<C2<float> *>(st), smem, sbo, tb, k, lowb, tid, -tsh);
+    else
+        move_tile<T, K, false, NT>(st, tile, sbo, tb, k, lowb, tid, -tsh);
 }
 
 // Order in which the workgroups of a pass take the 2^tbits tiles (tbits = nbits - K).  Workgroups are
@@ -770,6 +878,97 @@ __global__ __launch_bounds__(kNTD4, 4) void k_pass_d4(C2<float> *__restrict__ st
                                                       const OpDev<float> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
     pass_tile<float, K, true, true, kNTD4>(st, pd, gd, od, wd, tile_order(blockIdx.x, pd), smem);
+}
+
+// Persistent dense-only pass (4-qubit units; one QSV2 tile per workgroup tile, K = k, k - m >= 1):
+// workgroup b walks the tiles of logical index b, b + G, ... (G the grid).  The 16-byte HBM loads of
+// its next tile are issued into registers as soon as the current tile sits in LDS, so they are in
+// flight while the current tile's dense units run and while it is written back; the kernel keeps
+// two workgroups per CU.  Same arithmetic as k_pass_d4 on every tile.
+template <int K>
+__global__ __launch_bounds__(kNTD4, 4) void k_pass_d4p(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
+                                                       const GroupDev *__restrict__ gd, const float *__restrict__ wd) {
+    constexpr int NT = kNTD4;
+    constexpr uint32_t NE = 1u << K, NEP = NE + (NE >> 5);
+    constexpr int IT = NE / (2 * NT);   // 16-byte accesses per thread and tile
+    extern __shared__ __align__(16) unsigned char smem[];
+    C2<float> *tile = reinterpret_cast<C2<float> *>(smem);
+    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + NEP * sizeof(C2<float>));
+    const int m = pd->m, lowb = K - m;
+    uint32_t *red = reinterpret_cast<uint32_t *>(sbo + (1u << m));
+    const uint32_t tid = threadIdx.x, lowmask = (1u << lowb) - 1u;
+    for (uint32_t s = tid; s < (1u << m); s += NT) {
+        uint64_t o = 0;
+        for (int i = 0; i < m; ++i)
+            if ((s >> i) & 1u) o |= uint64_t(1) << pd->h[i];
+        sbo[s] = o;
+    }
+    __syncthreads();
+    const uint64_t ntiles = uint64_t(1) << (pd->nbits - K);
+    const uint64_t G = gridDim.x;
+    // element offset of the tile at logical index lb (uniform: scalar registers)
+    auto tile_base = [&](uint64_t lb) -> uint64_t {
+        uint64_t x = tile_order(lb, pd) << lowb;
+        for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
+        return x;
+    };
+    float4 v[IT];
+    // tt: the thread index, re-read per tile so the per-access offsets are recomputed (a few VALU)
+    // instead of being held in registers across the tile loop next to the prefetched tile
+    auto issue = [&](uint64_t tb, uint32_t tt) {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t e = 2u * (i * NT + tt);
+            v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * (tb | sbo[e >> lowb] | (e & lowmask)));
+        }
+    };
+    uint64_t b = blockIdx.x;
+    if (b >= ntiles) return;   // the host launches at most one workgroup per tile
+    uint64_t tb = tile_base(b);
+    issue(tb, tid);
+    const int ng = pd->ngroups, g0 = pd->group0, target = pd->tscale;
+    for (; b < ntiles; b += G) {
+        uint32_t tt = tid;
+        asm volatile("" : "+v"(tt));
+        int sh = 0;
+        if (target > 0) {
+            uint32_t mb = 0;
+#pragma unroll
+            for (int i = 0; i < IT; ++i)
+                mb = max(mb, max(max(absbits(v[i].x), absbits(v[i].y)), max(absbits(v[i].z), absbits(v[i].w))));
+            sh = tile_shift<NT>(mb, red, tid, target);   // its barrier also orders the last tile's reads
+        } else {
+            __syncthreads();   // every wave has read the previous tile out of LDS
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t pe = lpad(2u * (i * NT + tid));
+            tile[pe] = C2<float>{ldexpf(v[i].x, sh), ldexpf(v[i].y, sh)};
+            tile[pe + 1] = C2<float>{ldexpf(v[i].z, sh), ldexpf(v[i].w, sh)};
+        }
+        __syncthreads();
+        const uint64_t nb = b + G;
+        const uint64_t tbn = nb < ntiles ? tile_base(nb) : 0;
+        if (nb < ntiles) issue(tbn, tt);
+        for (int g = 0; g < ng; ++g) {
+            dense_group4<K, NT / 64>(smem, gd[g0 + g], wd, tid);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t e = 2u * (i * NT + tt), pe = lpad(e);
+            const C2<float> p0 = tile[pe], p1 = tile[pe + 1];
+            float4 w = make_float4(p0.x, p0.y, p1.x, p1.y);
+            if (sh) {
+                w.x = ldexpf(w.x, -sh);
+                w.y = ldexpf(w.y, -sh);
+                w.z = ldexpf(w.z, -sh);
+                w.w = ldexpf(w.w, -sh);
+            }
+            st_stream(reinterpret_cast<float *>(st) + 2 * (tb | sbo[e >> lowb] | (e & lowmask)), w);
+        }
+        tb = tbn;
+    }
 }
 
 // Continuous batching: one launch runs one pass of each of several states (the items, one kernel
@@ -1192,6 +1391,16 @@ hipError_t launch_pass_d4(int K, void *st, const PassDev *pd, const GroupDev *gd
     }
     return hipGetLastError();
 }
+hipError_t launch_pass_d4p(int K, void *st, const PassDev *pd, const GroupDev *gd, const float *wd, uint32_t grid,
+                           size_t lds, hipStream_t s) {
+    C2<float> *p = reinterpret_cast<C2<float> *>(st);
+    switch (K) {
+    case 12: hipLaunchKernelGGL((k_pass_d4p<12>), dim3(grid), dim3(kNTD4), lds, s, p, pd, gd, wd); break;
+    case 13: hipLaunchKernelGGL((k_pass_d4p<13>), dim3(grid), dim3(kNTD4), lds, s, p, pd, gd, wd); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 // batched launches of dense-only passes of 4-qubit units (the 512-thread kernel's passes)
 template <int K>
@@ -1563,6 +1772,7 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<float, 10, true>)) SVX_ATTR((k_pass<float, 11, true>)) SVX_ATTR((k_pass<float, 12, true>))
     SVX_ATTR((k_pass<float, 13, true>)) SVX_ATTR((k_pass<float, 14, true>))
     SVX_ATTR((k_pass_d4<12>)) SVX_ATTR((k_pass_d4<13>)) SVX_ATTR((k_pass_batch_d4<12>)) SVX_ATTR((k_pass_batch_d4<13>))
+    SVX_ATTR((k_pass_d4p<12>)) SVX_ATTR((k_pass_d4p<13>))
     SVX_ATTR((k_pass_batch<10, false>)) SVX_ATTR((k_pass_batch<10, true>)) SVX_ATTR((k_pass_batch<11, false>))
     SVX_ATTR((k_pass_batch<11, true>)) SVX_ATTR((k_pass_batch<12, false>)) SVX_ATTR((k_pass_batch<12, true>))
     SVX_ATTR((k_pass_batch<13, false>)) SVX_ATTR((k_pass_batch<13, true>)) SVX_ATTR((k_pass_batch<14, false>))
