@@ -240,7 +240,6 @@ struct LaunchInfo {
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
-    bool d4p;             // ... in its persistent form (next tile's loads in flight)
     int32_t stats[5];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32)
 };
 
@@ -658,11 +657,24 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 for (int b = 0; (int)qs.size() < F && b < K; ++b)
                     if (std::find(qs.begin(), qs.end(), b) == qs.end()) qs.push_back(b);
                 if ((int)qs.size() < F) return fail(SVX_E_CAPACITY, "tile too small for a dense op");
+                // tile bit 0 inside the unit becomes the matrix's index bit 0 (the 16-byte LDS path of
+                // the dense-only kernel pairs components 2c, 2c + 1): swap index bits 0 and p0
+                int p0 = 0;
+                for (int q = 0; q < F; ++q)
+                    if (qs[q] == 0) p0 = q;
+                std::swap(qs[0], qs[p0]);
+                auto sw = [p0](int x) {
+                    const int b0 = x & 1, bp = (x >> p0) & 1;
+                    return (x & ~1 & ~(1 << p0)) | (bp) | (b0 << p0);
+                };
                 for (int q = 0; q < F; ++q) gd.dq[q] = gd.dqs[q] = qs[q];
                 std::sort(gd.dqs, gd.dqs + F);
-                // padded matrix: the extra qubits are the high index bits (identity on them)
+                // padded matrix: the extra qubits are the high index bits (identity on them), in the
+                // index order before the bit-0 swap
                 const int D = 1 << F, d0 = 1 << f0;
                 auto U = [&](int r, int c, int part) -> double {
+                    r = sw(r);
+                    c = sw(c);
                     if ((r >> f0) != (c >> f0)) return 0.0;
                     return du.u[2 * ((r & (d0 - 1)) * d0 + (c & (d0 - 1))) + part];
                 };
@@ -797,12 +809,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             bool all4 = true;
             for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].f == 4;
             lr.d4 = all4;
-            // persistent form: one QSV2 tile per workgroup tile with a contiguous low bit, two
-            // workgroups per CU (SVX_D4=pers)
-            if (all4 && d4_mode == "pers" && K == k && k - rp.m >= 1) {
-                lr.d4p = true;
-                lr.grid = (uint32_t)std::min<uint64_t>(lr.ntiles, 2 * (uint64_t)ds->caps.cu_count);
-            }
         }
         for (int g = pd.group0; g < (int)groups.size(); ++g) {
             const GroupDev &G = groups[g];
@@ -895,10 +901,7 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
     for (const LaunchInfo &li : pr->launches) {
         if (li.empty) continue;
         hipError_t e;
-        if (li.d4p)
-            e = launch_pass_d4p(li.K, state, pd + li.pass_index, gd,
-                                reinterpret_cast<const float *>(base + pr->w_off), li.grid, li.lds, s);
-        else if (li.d4)
+        if (li.d4)
             e = launch_pass_d4(li.K, state, pd + li.pass_index, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
                                li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr, li.grid, li.lds, s);
         else if (li.pers)
@@ -1132,7 +1135,6 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
     else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
     else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5" : ",4") + (li.dense ? ",dense>" : ">");
-    else if (li.d4p) k = "k_pass_d4p<" + std::to_string(li.K) + ">";
     else if (li.d4) k = "k_pass_d4<" + std::to_string(li.K) + ">";
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());

@@ -92,9 +92,6 @@ hipError_t launch_pass_batch(int K, bool dense, bool d4, const BatchItem *items,
 // dense-only passes of 4-qubit units (512 threads), K in {12, 13}
 hipError_t launch_pass_d4(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
                           const float *wd, uint32_t grid, size_t lds, hipStream_t s);
-// persistent variant (K = k in {12, 13}, k - m >= 1, grid <= tiles): next tile's loads in flight
-hipError_t launch_pass_d4p(int K, void *st, const PassDev *pd, const GroupDev *gd, const float *wd, uint32_t grid,
-                           size_t lds, hipStream_t s);
 hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                            const OpDev<float> *od, const float *wd, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,

@@ -657,6 +657,143 @@ __device__ __forceinline__ void dense_apply_h(unsigned char *buf, const GroupDev
     }
 }
 
+// One 32-column batch of a 4-qubit split-FP16 unit in registers: x[ks][i] is component
+// 8 ks + 4 h + i of the lane's column; returns Y = W X (accumulator v, v + 1: re, im of component
+// 4 (v / 4) + 2 h + (v % 4) / 2), unscaled.
+template <bool PRE>
+__device__ __forceinline__ f16v unit4_h(const C2<float> (&x)[2][4], const half8 (&ah)[2], const half8 (&al)[2],
+                                        int wexp) {
+    int sh = 0;   // pre-scaled tile: every component is already below 2^15
+    if constexpr (!PRE) {
+        uint32_t mb = 0;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) mb = max(mb, max(absbits(x[ks][i].x), absbits(x[ks][i].y)));
+        mb = max(mb, (uint32_t)__shfl_xor((int)mb, 32));
+        const int efield = (int)(mb >> 23);
+        sh = mb == 0 ? 0 : 15 - (efield == 0 ? -125 : efield - 126);   // column maximum < 2^15
+    }
+    half8 bh[2], bl[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        u4v hv, lv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t hh, ll;
+            split_pk(PRE ? x[ks][i].x : ldexpf(x[ks][i].x, sh), PRE ? x[ks][i].y : ldexpf(x[ks][i].y, sh), hh, ll);
+            hv[i] = hh;
+            lv[i] = ll;
+        }
+        bh[ks] = __builtin_bit_cast(half8, hv);
+        bl[ks] = __builtin_bit_cast(half8, lv);
+    }
+    f16v acc = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[ks], bh[ks], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[ks], bl[ks], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[ks], bh[ks], acc, 0, 0, 0);
+    const int back = -(sh + wexp);
+    if (back != 0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] = ldexpf(acc[v], back);
+    }
+    return acc;
+}
+
+// 4-qubit split-FP16 unit on the swizzled tile with 16-byte LDS accesses (same arithmetic as
+// dense_apply_h).  Tile bit 0 inside the unit (the host makes it the matrix's index bit 0):
+// components 2c, 2c + 1 of a column share a 16-byte slot, two loads per K step.  Tile bit 0 outside:
+// a step covers 64 columns, the lane's columns 2j' and 2j' + 1 share every slot, and the two
+// 32-column batches of the step are those two halves.  Returns false (nothing done) when the tile
+// has too few column steps for the wavefronts; the caller then uses dense_apply_h.
+template <int K, int NWAVE, bool PRE>
+__device__ __forceinline__ bool dense4_h128(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                            uint32_t wt) {
+    constexpr uint32_t NCOL = 1u << (K - 4);
+    const bool inner = G.dq[0] == 0;
+    const uint32_t nstep = inner ? NCOL / 32 : NCOL / 64;
+    constexpr int NP = (NCOL / 32) % (2 * NWAVE) == 0 ? 2 : 1;   // batches per step, tile bit 0 inside
+    if (inner ? (nstep % (NP * NWAVE) != 0) : (nstep % NWAVE != 0)) return false;
+    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
+    const half8 *w8 = reinterpret_cast<const half8 *>(wd + G.w_off);
+    half8 ah[2], al[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        ah[ks] = w8[(ks * 2 + 0) * 64 + lane];
+        al[ks] = w8[(ks * 2 + 1) * 64 + lane];
+    }
+    ColOff<4, true> coff;
+    coff.init(G);
+    auto dep = [&](uint32_t c) {   // spread c over the tile bits outside the unit
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c = ins0_32(c, G.dqs[i]);
+        return c;
+    };
+    if (inner) {
+        const uint32_t pj = swz(dep(j));
+        for (uint32_t b = __builtin_amdgcn_readfirstlane(wave); b < nstep; b += NP * NWAVE) {
+            uint32_t pbl[NP], pbs[NP];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const uint32_t pb = pj ^ swz(dep((b + p * NWAVE) * 32u));
+                pbl[p] = pb ^ (h ? coff.lq[2] : 0u);
+                pbs[p] = pb ^ (h ? coff.lq[1] : 0u);
+            }
+            C2<float> x[NP][2][4];
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                    for (int i = 0; i < 4; i += 2) {   // components 8 ks + 4 h + i, + 1: one slot
+                        const float4 v = *reinterpret_cast<const float4 *>(buf + (pbl[p] ^ coff(8u * ks + i)));
+                        x[p][ks][i] = C2<float>{v.x, v.y};
+                        x[p][ks][i + 1] = C2<float>{v.z, v.w};
+                    }
+            f16v acc[NP];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) acc[p] = unit4_h<PRE>(x[p], ah, al, G.wexp);
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int v = 0; v < 16; v += 4)   // components 4 (v / 4) + 2 h, + 1: one slot
+                    *reinterpret_cast<float4 *>(buf + (pbs[p] ^ coff(4u * (v / 4)))) =
+                        make_float4(acc[p][v], acc[p][v + 1], acc[p][v + 2], acc[p][v + 3]);
+        }
+    } else {
+        const uint32_t pj = swz(dep(2u * j));
+        for (uint32_t b = __builtin_amdgcn_readfirstlane(wave); b < nstep; b += NWAVE) {
+            const uint32_t pb = pj ^ swz(dep(b * 64u));
+            const uint32_t pbl = pb ^ (h ? coff.lq[2] : 0u), pbs = pb ^ (h ? coff.lq[1] : 0u);
+            C2<float> x[2][2][4];
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {   // component 8 ks + 4 h + i of columns 2j', 2j' + 1
+                    const float4 v = *reinterpret_cast<const float4 *>(buf + (pbl ^ coff(8u * ks + i)));
+                    x[0][ks][i] = C2<float>{v.x, v.y};
+                    x[1][ks][i] = C2<float>{v.z, v.w};
+                }
+            // each column's result goes out as soon as it is ready (8-byte writes: LDS write
+            // throughput does not depend on the access width, and one accumulator set stays live)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const f16v acc = unit4_h<PRE>(x[p], ah, al, G.wexp);
+#pragma unroll
+                for (int v = 0; v < 16; v += 2) {
+                    const uint32_t c = 4u * (v / 4) + (v % 4) / 2;   // + 2 h: in pbs
+                    stc(buf, (pbs ^ coff(c)) + 8u * p, C2<float>{acc[v], acc[v + 1]});
+                }
+            }
+        }
+    }
+    return true;
+}
+
 // One dense group (type 1: FP32 matrix cores, 3: split FP16 with per-column scaling, 4: split FP16
 // on a pre-scaled tile) by NWAVE wavefronts
 template <int K, int NWAVE, bool SWZ>
@@ -670,7 +807,7 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
         else dense_apply_h<K, 4, NWAVE, SWZ, true>(buf, G, wd, wt);
     } else {
         if (G.f == 5) dense_apply_h<K, 5, NWAVE, SWZ, false>(buf, G, wd, wt);
-        else dense_apply_h<K, 4, NWAVE, SWZ, false>(buf, G, wd, wt);
+        else if (!SWZ || !dense4_h128<K, NWAVE, false>(buf, G, wd, wt)) dense_apply_h<K, 4, NWAVE, SWZ, false>(buf, G, wd, wt);
     }
 }
 
@@ -678,9 +815,13 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
 template <int K, int NWAVE, bool SWZ = false>
 __device__ __forceinline__ void dense_group4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                              uint32_t wt) {
-    if (G.type == 1) dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
-    else if (G.type == 4) dense_apply_h<K, 4, NWAVE, SWZ, true>(buf, G, wd, wt);
-    else dense_apply_h<K, 4, NWAVE, SWZ, false>(buf, G, wd, wt);
+    if (G.type == 1) {
+        dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
+    } else if (G.type == 4) {
+        if (!SWZ || !dense4_h128<K, NWAVE, true>(buf, G, wd, wt)) dense_apply_h<K, 4, NWAVE, SWZ, true>(buf, G, wd, wt);
+    } else {
+        if (!SWZ || !dense4_h128<K, NWAVE, false>(buf, G, wd, wt)) dense_apply_h<K, 4, NWAVE, SWZ, false>(buf, G, wd, wt);
+    }
 }
 
 // Dense-only passes keep the tile XOR-swizzled in LDS (swz: amplitudes 2i, 2i + 1 share one 16-byte
@@ -820,27 +961,24 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
             C2<T> a[NR];
 #pragma unroll
             for (int r = 0; r < NR; ++r) a[r] = tile[pb + poff[r]];
-            // the next op's descriptor (scalar loads) is in flight while this one is applied.
-            // Scalar loads return out of order, so the wait for the current descriptor has to
-            // come before the next one is issued: the empty asm reads it, the scheduling
-            // barrier keeps the next load behind that read.
-            OpDev<T> cur = od[o0];
 #pragma unroll 1
             for (int o = 0; o < nops; ++o) {
-                asm volatile("" ::"s"(cur.kind), "s"(cur.tr), "s"(cur.cmt), "s"(cur.cmr));
-                __builtin_amdgcn_sched_barrier(0);
-                const OpDev<T> nxt = od[o0 + min(o + 1, nops - 1)];
-                const bool ts = (base & cur.cmt) == cur.cmt;
-                apply_op<T, RR>(a, cur, ts);
-                cur = nxt;
+                const OpDev<T> &op = od[o0 + o];
+                const bool ts = (base & op.cmt) == op.cmt;
+                apply_op<T, RR>(a, op, ts);
             }
 #pragma unroll
             for (int r = 0; r < NR; ++r) tile[pb + poff[r]] = a[r];
         }
         __syncthreads();
     }
-    if constexpr (D4 && sizeof(T) == 4)
-        tile_out_swz<K, NT>(reinterpret_cast<C2<float> *>(st), smem, sbo, tb, k, lowb, tid, -tsh);
+    if constexpr (D4 && sizeof(T) == 4) {
+        // the store addresses are recomputed here rather than kept live (as common subexpressions
+        // of the load addresses) through the dense groups, which would spill registers
+        uint32_t tt = tid;
+        asm volatile("" : "+v"(tt));
+        tile_out_swz<K, NT>(reinterpret_cast<C2<float> *>(st), smem, sbo, tb, k, lowb, tt, -tsh);
+    }
     else
         move_tile<T, K, false, NT>(st, tile, sbo, tb, k, lowb, tid, -tsh);
 }
@@ -880,97 +1018,6 @@ __global__ __launch_bounds__(kNTD4, 4) void k_pass_d4(C2<float> *__restrict__ st
     pass_tile<float, K, true, true, kNTD4>(st, pd, gd, od, wd, tile_order(blockIdx.x, pd), smem);
 }
 
-// Persistent dense-only pass (4-qubit units; one QSV2 tile per workgroup tile, K = k, k - m >= 1):
-// workgroup b walks the tiles of logical index b, b + G, ... (G the grid).  The 16-byte HBM loads of
-// its next tile are issued into registers as soon as the current tile sits in LDS, so they are in
-// flight while the current tile's dense units run and while it is written back; the kernel keeps
-// two workgroups per CU.  Same arithmetic as k_pass_d4 on every tile.
-template <int K>
-__global__ __launch_bounds__(kNTD4, 4) void k_pass_d4p(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
-                                                       const GroupDev *__restrict__ gd, const float *__restrict__ wd) {
-    constexpr int NT = kNTD4;
-    constexpr uint32_t NE = 1u << K, NEP = NE + (NE >> 5);
-    constexpr int IT = NE / (2 * NT);   // 16-byte accesses per thread and tile
-    extern __shared__ __align__(16) unsigned char smem[];
-    C2<float> *tile = reinterpret_cast<C2<float> *>(smem);
-    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + NEP * sizeof(C2<float>));
-    const int m = pd->m, lowb = K - m;
-    uint32_t *red = reinterpret_cast<uint32_t *>(sbo + (1u << m));
-    const uint32_t tid = threadIdx.x, lowmask = (1u << lowb) - 1u;
-    for (uint32_t s = tid; s < (1u << m); s += NT) {
-        uint64_t o = 0;
-        for (int i = 0; i < m; ++i)
-            if ((s >> i) & 1u) o |= uint64_t(1) << pd->h[i];
-        sbo[s] = o;
-    }
-    __syncthreads();
-    const uint64_t ntiles = uint64_t(1) << (pd->nbits - K);
-    const uint64_t G = gridDim.x;
-    // element offset of the tile at logical index lb (uniform: scalar registers)
-    auto tile_base = [&](uint64_t lb) -> uint64_t {
-        uint64_t x = tile_order(lb, pd) << lowb;
-        for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
-        return x;
-    };
-    float4 v[IT];
-    // tt: the thread index, re-read per tile so the per-access offsets are recomputed (a few VALU)
-    // instead of being held in registers across the tile loop next to the prefetched tile
-    auto issue = [&](uint64_t tb, uint32_t tt) {
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const uint32_t e = 2u * (i * NT + tt);
-            v[i] = ld_stream(reinterpret_cast<const float *>(st) + 2 * (tb | sbo[e >> lowb] | (e & lowmask)));
-        }
-    };
-    uint64_t b = blockIdx.x;
-    if (b >= ntiles) return;   // the host launches at most one workgroup per tile
-    uint64_t tb = tile_base(b);
-    issue(tb, tid);
-    const int ng = pd->ngroups, g0 = pd->group0, target = pd->tscale;
-    for (; b < ntiles; b += G) {
-        uint32_t tt = tid;
-        asm volatile("" : "+v"(tt));
-        int sh = 0;
-        if (target > 0) {
-            uint32_t mb = 0;
-#pragma unroll
-            for (int i = 0; i < IT; ++i)
-                mb = max(mb, max(max(absbits(v[i].x), absbits(v[i].y)), max(absbits(v[i].z), absbits(v[i].w))));
-            sh = tile_shift<NT>(mb, red, tid, target);   // its barrier also orders the last tile's reads
-        } else {
-            __syncthreads();   // every wave has read the previous tile out of LDS
-        }
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const uint32_t pe = lpad(2u * (i * NT + tid));
-            tile[pe] = C2<float>{ldexpf(v[i].x, sh), ldexpf(v[i].y, sh)};
-            tile[pe + 1] = C2<float>{ldexpf(v[i].z, sh), ldexpf(v[i].w, sh)};
-        }
-        __syncthreads();
-        const uint64_t nb = b + G;
-        const uint64_t tbn = nb < ntiles ? tile_base(nb) : 0;
-        if (nb < ntiles) issue(tbn, tt);
-        for (int g = 0; g < ng; ++g) {
-            dense_group4<K, NT / 64>(smem, gd[g0 + g], wd, tid);
-            __syncthreads();
-        }
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const uint32_t e = 2u * (i * NT + tt), pe = lpad(e);
-            const C2<float> p0 = tile[pe], p1 = tile[pe + 1];
-            float4 w = make_float4(p0.x, p0.y, p1.x, p1.y);
-            if (sh) {
-                w.x = ldexpf(w.x, -sh);
-                w.y = ldexpf(w.y, -sh);
-                w.z = ldexpf(w.z, -sh);
-                w.w = ldexpf(w.w, -sh);
-            }
-            st_stream(reinterpret_cast<float *>(st) + 2 * (tb | sbo[e >> lowb] | (e & lowmask)), w);
-        }
-        tb = tbn;
-    }
-}
-
 // Continuous batching: one launch runs one pass of each of several states (the items, one kernel
 // shape per launch); workgroup b runs tile b - items[i].tile0 of the item i whose tile range holds
 // b, with the same device code as k_pass, so every state gets the same arithmetic alone and in
@@ -985,7 +1032,7 @@ __global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass_batch(cons
         if (items[mid].tile0 <= b) lo = mid;
         else hi = mid - 1;
     }
-    const BatchItem &it = items[lo];
+    const BatchItem &it = items[__builtin_amdgcn_readfirstlane(lo)];   // uniform: scalar loads
     pass_tile<float, K, DN>(reinterpret_cast<C2<float> *>(it.st), it.pd, it.gd,
                             reinterpret_cast<const OpDev<float> *>(it.od), it.wd, tile_order(b - it.tile0, it.pd), smem);
 }
@@ -1391,17 +1438,6 @@ hipError_t launch_pass_d4(int K, void *st, const PassDev *pd, const GroupDev *gd
     }
     return hipGetLastError();
 }
-hipError_t launch_pass_d4p(int K, void *st, const PassDev *pd, const GroupDev *gd, const float *wd, uint32_t grid,
-                           size_t lds, hipStream_t s) {
-    C2<float> *p = reinterpret_cast<C2<float> *>(st);
-    switch (K) {
-    case 12: hipLaunchKernelGGL((k_pass_d4p<12>), dim3(grid), dim3(kNTD4), lds, s, p, pd, gd, wd); break;
-    case 13: hipLaunchKernelGGL((k_pass_d4p<13>), dim3(grid), dim3(kNTD4), lds, s, p, pd, gd, wd); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 // batched launches of dense-only passes of 4-qubit units (the 512-thread kernel's passes)
 template <int K>
 __global__ __launch_bounds__(kNTD4, 4) void k_pass_batch_d4(const BatchItem *__restrict__ items, int n_items) {
@@ -1413,7 +1449,7 @@ __global__ __launch_bounds__(kNTD4, 4) void k_pass_batch_d4(const BatchItem *__r
         if (items[mid].tile0 <= b) lo = mid;
         else hi = mid - 1;
     }
-    const BatchItem &it = items[lo];
+    const BatchItem &it = items[__builtin_amdgcn_readfirstlane(lo)];   // uniform: scalar loads
     pass_tile<float, K, true, true, kNTD4>(reinterpret_cast<C2<float> *>(it.st), it.pd, it.gd,
                                            reinterpret_cast<const OpDev<float> *>(it.od), it.wd,
                                            tile_order(b - it.tile0, it.pd), smem);
@@ -1772,7 +1808,6 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<float, 10, true>)) SVX_ATTR((k_pass<float, 11, true>)) SVX_ATTR((k_pass<float, 12, true>))
     SVX_ATTR((k_pass<float, 13, true>)) SVX_ATTR((k_pass<float, 14, true>))
     SVX_ATTR((k_pass_d4<12>)) SVX_ATTR((k_pass_d4<13>)) SVX_ATTR((k_pass_batch_d4<12>)) SVX_ATTR((k_pass_batch_d4<13>))
-    SVX_ATTR((k_pass_d4p<12>)) SVX_ATTR((k_pass_d4p<13>))
     SVX_ATTR((k_pass_batch<10, false>)) SVX_ATTR((k_pass_batch<10, true>)) SVX_ATTR((k_pass_batch<11, false>))
     SVX_ATTR((k_pass_batch<11, true>)) SVX_ATTR((k_pass_batch<12, false>)) SVX_ATTR((k_pass_batch<12, true>))
     SVX_ATTR((k_pass_batch<13, false>)) SVX_ATTR((k_pass_batch<13, true>)) SVX_ATTR((k_pass_batch<14, false>))
