@@ -778,16 +778,16 @@ __device__ __forceinline__ bool dense4_h128(unsigned char *buf, const GroupDev &
                     x[0][ks][i] = C2<float>{v.x, v.y};
                     x[1][ks][i] = C2<float>{v.z, v.w};
                 }
-            // each column's result goes out as soon as it is ready (8-byte writes: LDS write
-            // throughput does not depend on the access width, and one accumulator set stays live)
+            // both columns' results of a component go out as one 16-byte write (8-byte writes of
+            // one column would all fall in the same half of their slots)
+            f16v acc[2];
 #pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const f16v acc = unit4_h<PRE>(x[p], ah, al, G.wexp);
+            for (int p = 0; p < 2; ++p) acc[p] = unit4_h<PRE>(x[p], ah, al, G.wexp);
 #pragma unroll
-                for (int v = 0; v < 16; v += 2) {
-                    const uint32_t c = 4u * (v / 4) + (v % 4) / 2;   // + 2 h: in pbs
-                    stc(buf, (pbs ^ coff(c)) + 8u * p, C2<float>{acc[v], acc[v + 1]});
-                }
+            for (int v = 0; v < 16; v += 2) {
+                const uint32_t c = 4u * (v / 4) + (v % 4) / 2;   // + 2 h: in pbs
+                *reinterpret_cast<float4 *>(buf + (pbs ^ coff(c))) =
+                    make_float4(acc[0][v], acc[0][v + 1], acc[1][v], acc[1][v + 1]);
             }
         }
     }
