@@ -331,9 +331,10 @@ def test_pass_fusion_double_buffered_kernels(gpu, n, k, pick, monkeypatch):
 def test_dense_only_kernel_matches_general_kernel(gpu, n, k, monkeypatch):
     """Every op of every pass in a dense 4-qubit unit (SVX_FUSE=dense4) runs on the 512-thread
     dense-only kernel; it is bit-identical to the general dense kernel (SVX_D4=off: same units, same
-    arithmetic, other thread count) and matches the complex128 golden model."""
+    arithmetic, other thread count; unit pairs off) and matches the complex128 golden model."""
     from u55c_statevector_30q_amd import libsvx
     monkeypatch.setenv("SVX_FUSE", "dense4")
+    monkeypatch.setenv("SVX_PAIR", "off")
     rng = np.random.default_rng(3 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)
     s0 = _rand_state(rng, n)
@@ -348,13 +349,82 @@ def test_dense_only_kernel_matches_general_kernel(gpu, n, k, monkeypatch):
     assert np.array_equal(got["auto"], got["off"]), names
 
 
+def _dense_pass_words(rng, n, k, hi, unit_qubits):
+    """One pass of fused-form dense records (random 16 x 16 unitaries) on the given tile qubits."""
+    from u55c_statevector_30q_amd import compiler
+    w = [MAGIC, n, 1, k, 0, 0, 0, 0]
+    m = len(hi)
+    h = list(hi) + [0] * (11 - m)
+    w += [m, h[0], h[1], len(unit_qubits), h[2], h[3], 0, 0]
+    if m > 4:
+        w += [0x48] + h[4:11]
+    for qs in unit_qubits:
+        z = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+        q, r = np.linalg.qr(z)
+        w += compiler.dense_words(q * (np.diag(r) / np.abs(np.diag(r))), qs)
+    return (np.asarray(w, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+
+
+def _pair_runs(libsvx, seg, n, s0, monkeypatch):
+    got, pairs = {}, {}
+    for pm in ("on", "off"):
+        monkeypatch.setenv("SVX_PAIR", pm)
+        h = libsvx.prepare(seg, libsvx.C64, n)
+        pairs[pm] = sum(libsvx.prepared_pass_stats(h, p)["dense_pairs"] for p in range(libsvx.prepared_info(h)[1]))
+        got[pm] = run_hip(s0.astype(np.complex64), seg)
+    return got, pairs
+
+
+@pytest.mark.parametrize("n,k,max_m", [(20, 13, 6), (22, 13, 7), (21, 12, 5)])
+def test_dense_unit_pairs_match_oracle(gpu, n, k, max_m, monkeypatch):
+    """SVX_PAIR=on: consecutive pre-scaled 4-qubit units on disjoint bits share one LDS round trip on
+    the dense-only kernel (unit A transposed on the matrix cores, its accumulators turned into unit B's
+    operand by 16- and 32-lane register swaps): pairs are formed, the result matches the complex128
+    golden model and the unpaired form to the complex64 bar."""
+    from u55c_statevector_30q_amd import libsvx
+    monkeypatch.setenv("SVX_FUSE", "dense4")
+    rng = np.random.default_rng(11 * n + k + max_m)
+    seg = random_segment(rng, n, k, n_passes=4, max_ops=36, max_m=max_m, ctrl_p=0.05)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got, pairs = _pair_runs(libsvx, seg, n, s0, monkeypatch)
+    assert pairs["on"] > 0 and pairs["off"] == 0, pairs
+    for pm in got:
+        _check_c64(ref, got[pm], (n, k, max_m, pm))
+    m = judge.metrics(got["off"].astype(np.complex128), got["on"])
+    assert m["one_minus_f"] < 1e-9, m
+
+
+@pytest.mark.parametrize("units", [
+    [(0, 1, 2, 3), (4, 5, 6, 7)],                          # tile bit 0 in the first unit: 16-byte loads
+    [(5, 6, 7, 8), (9, 10, 11, 12)],                       # tile bit 0 a context bit
+    [(3, 0, 9, 6), (12, 2, 7, 10), (1, 4, 5, 8), (11, 0, 2, 3)],   # two pairs, bit 0 inside, unordered bits
+    [(9, 10, 11, 12), (1, 2, 3, 4), (0, 5, 6, 7)],         # a pair, then a single unit
+])
+def test_dense_unit_pairs_on_chosen_bits(gpu, units, monkeypatch):
+    """Fused-form dense records on chosen tile bits (20 qubits, 2^13 tiles, 4 high qubits): both load
+    paths of the paired form, pairs followed by a single unit, bits in any order; vs the golden model."""
+    from u55c_statevector_30q_amd import libsvx
+    n, k = 20, 13
+    rng = np.random.default_rng(sum(sum(u) for u in units))
+    seg = _dense_pass_words(rng, n, k, [16, 17, 18, 19], units)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got, pairs = _pair_runs(libsvx, seg, n, s0, monkeypatch)
+    assert pairs["on"] >= 1 and pairs["off"] == 0, pairs
+    for pm in got:
+        m = judge.metrics(ref, got[pm])
+        assert m["one_minus_f"] < 1e-9 and m["max_abs_err"] < 2e-6, (pm, m)
+
+
 @pytest.mark.parametrize("tile_scale", ["on", "off"])
 @pytest.mark.parametrize("n,k", [(24, 13), (23, 12)])
 def test_dense_only_kernel_large_states(gpu, n, k, tile_scale, monkeypatch):
     """The dense-only kernel on states of 2^11 tiles, with pre-scaled tiles and with per-column
-    scaling: bit-identical to the general dense kernel (SVX_D4=off), norm kept."""
+    scaling: bit-identical to the general dense kernel (SVX_D4=off; unit pairs off), norm kept."""
     from u55c_statevector_30q_amd import libsvx
     monkeypatch.setenv("SVX_FUSE", "dense4")
+    monkeypatch.setenv("SVX_PAIR", "off")
     monkeypatch.setenv("SVX_TILE_SCALE", tile_scale)
     rng = np.random.default_rng(5 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)

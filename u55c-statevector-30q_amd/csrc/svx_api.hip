@@ -240,7 +240,7 @@ struct LaunchInfo {
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
-    int32_t stats[5];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32)
+    int32_t stats[6];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32), pairs
 };
 
 struct Prepared {
@@ -470,6 +470,11 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     }
     const char *d4env = std::getenv("SVX_D4");
     const std::string d4_mode = d4env ? d4env : "auto";
+    // pairs of 4-qubit units on disjoint bits in one LDS round trip (dense-only passes, SVX_PAIR=on): the
+    // paired form measured slower than two separate units on the 30-qubit workload (4.06 vs 3.75 ms per
+    // dense-only pass), so it is opt-in
+    const char *penv = std::getenv("SVX_PAIR");
+    const bool pair_ok = penv && std::string(penv) == "on";
     const char *oenv = std::getenv("SVX_TILE_ORDER");
     const int torder = oenv ? std::atoi(oenv) : -2;   // default: each XCD streams a contiguous range of tiles
     if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
@@ -501,6 +506,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     std::vector<OpDev<T>> ops;
     std::vector<LaunchInfo> launches;
     std::vector<float> wmat;      // dense matrices, MFMA A-operand order
+    std::map<int, std::vector<double>> umat4;   // group index -> padded 16 x 16 matrix of a split-FP16 unit
     for (const RawPass &rp : seg.passes) {
         // classify
         std::vector<OpDev<T>> pops;
@@ -727,6 +733,15 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     const size_t o = wmat.size();
                     wmat.resize(o + hb.size() / 2);
                     std::memcpy(wmat.data() + o, hb.data(), hb.size() * sizeof(uint16_t));
+                    if (F == 4) {   // kept for the paired form
+                        std::vector<double> um(size_t(2) * D * D);
+                        for (int r = 0; r < D; ++r)
+                            for (int c = 0; c < D; ++c) {
+                                um[2 * (size_t(r) * D + c)] = U(r, c, 0);
+                                um[2 * (size_t(r) * D + c) + 1] = U(r, c, 1);
+                            }
+                        umat4[(int)groups.size()] = std::move(um);
+                    }
                 }
                 // max-abs growth bound of the op: largest row sum of |U| (padding adds identity rows)
                 double rs_max = 0;
@@ -809,6 +824,47 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             bool all4 = true;
             for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].f == 4;
             lr.d4 = all4;
+            // consecutive pre-scaled split-FP16 units on disjoint bits become pairs (one LDS round trip;
+            // the first of a pair gets type 5), their operands in the paired (B-operand) order:
+            // block (ks 2 + part), lane l, entry e = Q[16 ks + 8 (l / 32) + e][l % 32] with
+            // Q[(p, a)][(p', a')] = [[Re U, Im U], [-Im U, Re U]][p][p'] at U[a'][a]
+            if (all4 && pair_ok && pd.tscale > 0) {
+                auto pack_pair = [&](GroupDev &G, const std::vector<double> &um) {
+                    const double wsc = std::ldexp(1.0, G.wexp);
+                    while (wmat.size() % 4) wmat.push_back(0.f);
+                    G.w_off = (int64_t)wmat.size();
+                    std::vector<uint16_t> hb;
+                    hb.reserve(2 * 2 * 512);
+                    for (int ks = 0; ks < 2; ++ks)
+                        for (int part = 0; part < 2; ++part)
+                            for (int l = 0; l < 64; ++l)
+                                for (int e = 0; e < 8; ++e) {
+                                    const int a = 8 * (l / 32) + e, pp = (l % 32) / 16, ap = l % 16;
+                                    const double re = um[2 * (size_t(ap) * 16 + a)], im = um[2 * (size_t(ap) * 16 + a) + 1];
+                                    const double q = (ks == 0 ? (pp == 0 ? re : im) : (pp == 0 ? -im : re)) * wsc;
+                                    const uint16_t hh = f32_to_f16_bits((float)q);
+                                    hb.push_back(part == 0 ? hh : f32_to_f16_bits((float)(q - f16_bits_to_f32(hh))));
+                                }
+                    const size_t o = wmat.size();
+                    wmat.resize(o + hb.size() / 2);
+                    std::memcpy(wmat.data() + o, hb.data(), hb.size() * sizeof(uint16_t));
+                };
+                for (int g = pd.group0; g + 1 < (int)groups.size();) {
+                    GroupDev &A = groups[g], &B = groups[g + 1];
+                    bool ok = A.type == 4 && B.type == 4 && A.f == 4 && B.f == 4 && umat4.count(g) && umat4.count(g + 1);
+                    if (ok)
+                        for (int i = 0; i < 4; ++i)
+                            for (int j = 0; j < 4; ++j) ok = ok && A.dq[i] != B.dq[j];
+                    if (!ok) {
+                        ++g;
+                        continue;
+                    }
+                    pack_pair(A, umat4[g]);
+                    pack_pair(B, umat4[g + 1]);
+                    A.type = 5;
+                    g += 2;
+                }
+            }
         }
         for (int g = pd.group0; g < (int)groups.size(); ++g) {
             const GroupDev &G = groups[g];
@@ -818,6 +874,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             } else {
                 lr.stats[G.f == 5 ? 3 : 2] += 1;
                 lr.stats[4] += G.type == 1;
+                lr.stats[5] += G.type == 5;
             }
         }
         passes.push_back(pd);
@@ -855,9 +912,10 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     const char *tenv = std::getenv("SVX_TILE_SCALE");
     const char *d4env = std::getenv("SVX_D4");
     const char *oenv = std::getenv("SVX_TILE_ORDER");
+    const char *penv = std::getenv("SVX_PAIR");
     const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
                              (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "") + "/" +
-                             (d4env ? d4env : "") + "/" + (oenv ? oenv : "");
+                             (d4env ? d4env : "") + "/" + (oenv ? oenv : "") + "/" + (penv ? penv : "");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);
@@ -1147,7 +1205,7 @@ int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out) {
         return fail(SVX_E_ARG, "unknown prepared-segment handle");
     Prepared *pr = g_prepared[handle - 1].get();
     if (pass < 0 || pass >= (int32_t)pr->launches.size() || !out) return fail(SVX_E_ARG, "bad pass index");
-    for (int i = 0; i < 5; ++i) out[i] = pr->launches[pass].stats[i];
+    for (int i = 0; i < 6; ++i) out[i] = pr->launches[pass].stats[i];
     return 0;
 }
 

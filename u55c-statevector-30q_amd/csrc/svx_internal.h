@@ -47,7 +47,9 @@ struct GroupDev {
     int32_t bs[kR];              // b sorted ascending (slot deposit)
     int32_t type;                // 0: MUX2 register group; 1: dense op, FP32 MFMA; 2: dense op, small-state
                                  // kernel; 3: dense op, split-FP16 MFMA, per-column scaling; 4: the same
-                                 // on a tile pre-scaled by the pass kernel (PassDev::tscale)
+                                 // on a tile pre-scaled by the pass kernel (PassDev::tscale); 5: first of
+                                 // a pair of 4-qubit type-4 units on disjoint bits applied in one LDS
+                                 // round trip (the next group is the second; operands in pair order)
     int32_t f;                   // dense: qubits of the op (type 1: 4 or 5 after padding)
     int32_t dq[5];               // dense: workgroup-tile bit of the op's qubit i (matrix index bit i)
     int32_t dqs[5];              // dq sorted ascending (column deposit)
