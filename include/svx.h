@@ -87,7 +87,8 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
 /* Work of pass `pass` of a prepared segment after classification and pass fusion:
  * out[0] MUX2 register groups, out[1] MUX2 ops, out[2] dense units on 4 qubits, out[3] dense units on
  * 5 qubits, out[4] dense units on the exact FP32 matrix-core path, out[5] pairs of 4-qubit units applied
- * in one LDS round trip (out has 6 entries).  No reference counterpart. */
+ * in one LDS round trip, out[6] diagonal tables (runs of diagonal ops applied as one table of phases)
+ * (out has 7 entries).  No reference counterpart. */
 int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out);
 
 /* Qubit permutation perm[new_bit] = old_bit of a 2^n state, out of place (dst != src), bit-exact.

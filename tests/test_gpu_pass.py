@@ -222,20 +222,22 @@ def test_prepared_replay_is_identical(gpu):
 
 @pytest.mark.parametrize("n,k,max_m", [(21, 12, 4), (21, 12, 7), (22, 13, 7), (22, 13, 8)])
 def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m, monkeypatch):
-    """The double-buffered kernel (one QSV2 tile per LDS buffer, loader wave + worker waves) and
-    the single-buffer kernel apply the same ops in the same order with the same arithmetic: their
-    results are bit-identical, and both match the complex128 golden model."""
+    """The double-buffered kernel (one QSV2 tile per LDS buffer, loader wave + worker waves), the
+    persistent MUX2 kernel and the single-buffer kernel apply the same ops in the same order with the
+    same arithmetic: their results are bit-identical, and all match the complex128 golden model."""
     rng = np.random.default_rng(n * 100 + k + max_m)
     seg = random_segment(rng, n, k, n_passes=2, max_ops=10, max_m=max_m)
     s0 = _rand_state(rng, n, np.complex64)
     ref = model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128)
     got = {}
     monkeypatch.setenv("SVX_FUSE", "off")
-    for pick in ("dma", "v1"):
+    monkeypatch.setenv("SVX_DIAG", "off")   # diagonal tables: single-buffer kernel only
+    for pick in ("dma", "pers", "v1"):
         monkeypatch.setenv("SVX_PASS_KERNEL", pick)
         got[pick] = run_hip(s0, seg)
         _check_c64(ref, got[pick], (n, k, max_m, pick))
     assert np.array_equal(got["dma"], got["v1"])
+    assert np.array_equal(got["pers"], got["v1"])
 
 
 @pytest.mark.parametrize("fuse,cost", [("off", None), ("auto", None), ("on", None), ("on", "0,0,5"),

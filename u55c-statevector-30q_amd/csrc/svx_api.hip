@@ -240,7 +240,8 @@ struct LaunchInfo {
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
-    int32_t stats[6];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32), pairs
+    int32_t stats[7];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32), pairs,
+                          // diagonal tables
 };
 
 struct Prepared {
@@ -274,10 +275,17 @@ struct DenseUnit {
     int f = 0;
     int q[5] = {0, 0, 0, 0, 0};
     std::vector<double> u;
+    // diagonal table (diag = true): the product of a run of diagonal ops over the tile bits dmask,
+    // u = 2^k (re, im) entries indexed by the QSV2 tile index
+    bool diag = false;
+    uint32_t dmask = 0;
 };
 
 // VALU cost of one op per amplitude, in FMAs (a general 2x2 MUX2 = 8), by OpKind
 const double kOpCost[] = {8, 4, 1, 8, 4, 2, 2, 8, 2, 2};
+// smallest MUX2 cost of a run of diagonal ops that becomes a table (one complex multiply and one
+// table read per amplitude, plus an LDS round trip)
+constexpr double kDiagRunCost = 8.0;
 
 struct FuseCfg {
     bool all_dense = false; // every block (also single ops) becomes a dense unit
@@ -343,7 +351,7 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
     for (int i = 0; i < n; ++i) {
         if (dense_of[i] >= 0) {
             const DenseUnit &du = dus[dense_of[i]];
-            uint32_t m = 0;
+            uint32_t m = du.dmask;
             for (int q = 0; q < du.f; ++q) m |= 1u << du.q[q];
             qm[i] = m;
             fusable[i] = 0;
@@ -435,6 +443,96 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
     dense_of.swap(out_dense);
 }
 
+// Diagonal tables (complex64): diagonal ops commute with each other, so a run of them -- hoisted
+// over the ops between them that share no bit with them -- is one diagonal operator over the QSV2
+// tile index, applied as a table of 2^k phases (one complex multiply per amplitude instead of the
+// run's MUX2 work).  Scanning in pass order, a diagonal op joins the open run unless it shares a bit
+// with an op already deferred behind the run; other ops are deferred (kept in order, after the
+// run).  A run of at least two ops whose MUX2 cost reaches min_cost becomes a table, computed in
+// FP64 from the ops' complex64 entries with the reference's selection rule (pair index i0 with the
+// target bit clear, M1 iff cmask != 0 and (i0 & cmask) == cmask: host/sv/model.py:43-56).
+template <typename T>
+void diag_runs(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vector<DenseUnit> &dus, int k,
+               double min_cost) {
+    auto bits = [&](size_t i) -> uint32_t {
+        if (dense_of[i] < 0) return (1u << pops[i].t) | (uint32_t)pops[i].cmask;
+        const DenseUnit &du = dus[dense_of[i]];
+        uint32_t m = du.dmask;
+        for (int q = 0; q < du.f; ++q) m |= 1u << du.q[q];
+        return m;
+    };
+    auto diag = [&](size_t i) {
+        return dense_of[i] < 0 && is_diag(pops[i].m0) && (pops[i].cmask == 0 || is_diag(pops[i].m1));
+    };
+    std::vector<OpDev<T>> out;
+    std::vector<int> out_dense;
+    std::vector<size_t> run, def;
+    uint32_t mrun = 0, mdef = 0;
+    double rcost = 0;
+    auto flush = [&]() {
+        if (run.size() >= 2 && rcost >= min_cost) {
+            const size_t D = size_t(1) << k;
+            DenseUnit du;
+            du.diag = true;
+            du.dmask = mrun;
+            du.u.assign(2 * D, 0.0);
+            for (size_t x = 0; x < D; ++x) {
+                double re = 1.0, im = 0.0;
+                for (size_t i : run) {
+                    const OpDev<T> &o = pops[i];
+                    const uint32_t i0 = (uint32_t)x & ~(1u << o.t);
+                    const bool sel = o.cmask != 0 && (i0 & (uint32_t)o.cmask) == (uint32_t)o.cmask;
+                    const T *M = sel ? o.m1 : o.m0;
+                    const int e = ((x >> o.t) & 1) ? 6 : 0;   // m11 or m00
+                    const double dr = (double)M[e], di = (double)M[e + 1];
+                    const double nr = re * dr - im * di;
+                    im = re * di + im * dr;
+                    re = nr;
+                }
+                du.u[2 * x] = re;
+                du.u[2 * x + 1] = im;
+            }
+            OpDev<T> od{};
+            od.kind = -1;
+            od.t = __builtin_ctz(mrun);
+            out.push_back(od);
+            out_dense.push_back((int)dus.size());
+            dus.push_back(std::move(du));
+        } else {
+            for (size_t i : run) {
+                out.push_back(pops[i]);
+                out_dense.push_back(-1);
+            }
+        }
+        for (size_t i : def) {
+            out.push_back(pops[i]);
+            out_dense.push_back(dense_of[i]);
+        }
+        run.clear();
+        def.clear();
+        mrun = mdef = 0;
+        rcost = 0;
+    };
+    for (size_t i = 0; i < pops.size(); ++i) {
+        const uint32_t b = bits(i);
+        if (diag(i)) {
+            if (b & mdef) flush();
+            run.push_back(i);
+            mrun |= b;
+            rcost += kOpCost[pops[i].kind];
+        } else if (run.empty()) {
+            out.push_back(pops[i]);
+            out_dense.push_back(dense_of[i]);
+        } else {
+            def.push_back(i);
+            mdef |= b;
+        }
+    }
+    flush();
+    pops.swap(out);
+    dense_of.swap(out_dense);
+}
+
 template <typename T>
 int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     const int k = seg.k;
@@ -477,6 +575,10 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     const bool pair_ok = penv && std::string(penv) == "on";
     const char *oenv = std::getenv("SVX_TILE_ORDER");
     const int torder = oenv ? std::atoi(oenv) : -2;   // default: each XCD streams a contiguous range of tiles
+    // runs of diagonal ops as one table each (complex64, single-buffer kernel; SVX_DIAG=off: never;
+    // SVX_FUSE=dense4 puts every op in a dense unit instead)
+    const char *genv = std::getenv("SVX_DIAG");
+    const bool diag_ok = !(genv && std::string(genv) == "off");
     if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
         double c4 = fcfg.cost4, c5 = fcfg.cost5;
         int wd = fcfg.width;
@@ -562,6 +664,10 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             }
             cost += kOpCost[od.kind];
             pops.push_back(od);
+        }
+        if (dtype == SVX_C64 && !small && diag_ok && fuse_mode != "dense4" && (pick == "auto" || pick == "v1")) {
+            diag_runs(pops, dense_of, dus, k, kDiagRunCost);
+            for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
         if (dtype == SVX_C64 && !small && fuse_mode != "off" &&
             (pick == "auto" || pick == "v1" || pick == "dma" || pick == "dma5")) {
@@ -654,6 +760,24 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         size_t i = 0;
         double log2_growth = 0;   // bound on the growth of the tile's largest |amplitude| over the pass
         while (i < pops.size()) {
+            if (dense_of[i] >= 0 && dus[dense_of[i]].diag) {
+                // diagonal table: 2^k complex64 entries (8-byte aligned), |entries| bound the growth
+                const DenseUnit &du = dus[dense_of[i]];
+                GroupDev gd{};
+                gd.type = 6;
+                while (wmat.size() % 4) wmat.push_back(0.f);
+                gd.w_off = (int64_t)wmat.size();
+                double amax = 0;
+                for (size_t x = 0; 2 * x < du.u.size(); ++x) {
+                    wmat.push_back((float)du.u[2 * x]);
+                    wmat.push_back((float)du.u[2 * x + 1]);
+                    amax = std::max(amax, std::hypot(du.u[2 * x], du.u[2 * x + 1]));
+                }
+                log2_growth += std::log2(std::max(1.0, amax));
+                groups.push_back(gd);
+                ++i;
+                continue;
+            }
             if (dense_of[i] >= 0) {
                 const DenseUnit &du = dus[dense_of[i]];
                 const int f0 = du.f, F = std::max(4, f0);
@@ -809,7 +933,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         // Tile scaling for the split-FP16 groups: the kernel scales each tile so its largest
         // component is below 2^tscale; 2^tscale times the pass's growth bound (and sqrt 2 for
         // re/im against complex magnitudes) stays below 2^15, the FP16 range the split needs.
-        if (mode == 1 && has_dense && tile_scale_ok) {
+        bool split16 = false;   // split-FP16 dense groups (the only ones that need the scaled tile)
+        for (int g = pd.group0; g < (int)groups.size(); ++g) split16 = split16 || groups[g].type == 3;
+        if (mode == 1 && split16 && tile_scale_ok) {
             const int h = (int)std::ceil(log2_growth + 0.5 + 0.02);
             if (15 - h >= 4) {
                 pd.tscale = 15 - h;
@@ -871,6 +997,8 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             if (G.type == 0) {
                 lr.stats[0] += 1;
                 lr.stats[1] += G.nops;
+            } else if (G.type == 6) {
+                lr.stats[6] += 1;
             } else {
                 lr.stats[G.f == 5 ? 3 : 2] += 1;
                 lr.stats[4] += G.type == 1;
@@ -913,9 +1041,11 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     const char *d4env = std::getenv("SVX_D4");
     const char *oenv = std::getenv("SVX_TILE_ORDER");
     const char *penv = std::getenv("SVX_PAIR");
+    const char *genv = std::getenv("SVX_DIAG");
     const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
                              (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "") + "/" +
-                             (d4env ? d4env : "") + "/" + (oenv ? oenv : "") + "/" + (penv ? penv : "");
+                             (d4env ? d4env : "") + "/" + (oenv ? oenv : "") + "/" + (penv ? penv : "") + "/" +
+                             (genv ? genv : "");
     const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
                                         (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
     std::lock_guard<std::mutex> lk(g_prep_mu);
@@ -1205,7 +1335,7 @@ int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out) {
         return fail(SVX_E_ARG, "unknown prepared-segment handle");
     Prepared *pr = g_prepared[handle - 1].get();
     if (pass < 0 || pass >= (int32_t)pr->launches.size() || !out) return fail(SVX_E_ARG, "bad pass index");
-    for (int i = 0; i < 6; ++i) out[i] = pr->launches[pass].stats[i];
+    for (int i = 0; i < 7; ++i) out[i] = pr->launches[pass].stats[i];
     return 0;
 }
 

@@ -49,7 +49,9 @@ struct GroupDev {
                                  // kernel; 3: dense op, split-FP16 MFMA, per-column scaling; 4: the same
                                  // on a tile pre-scaled by the pass kernel (PassDev::tscale); 5: first of
                                  // a pair of 4-qubit type-4 units on disjoint bits applied in one LDS
-                                 // round trip (the next group is the second; operands in pair order)
+                                 // round trip (the next group is the second; operands in pair order);
+                                 // 6: diagonal table, 2^k complex64 phases at w_off indexed by the QSV2
+                                 // tile index (a run of diagonal ops, complex64 single-buffer kernel)
     int32_t f;                   // dense: qubits of the op (type 1: 4 or 5 after padding)
     int32_t dq[5];               // dense: workgroup-tile bit of the op's qubit i (matrix index bit i)
     int32_t dqs[5];              // dq sorted ascending (column deposit)

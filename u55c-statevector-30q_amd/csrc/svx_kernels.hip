@@ -1036,6 +1036,22 @@ __device__ __forceinline__ void tile_out_swz(C2<float> *__restrict__ st, const u
 // One workgroup tile of one pass: `tile` selects the 2^(K-k) QSV2 tiles the workgroup owns.
 // DN: the pass has dense groups; D4: dense-only instance (every group a dense 4-qubit unit, NT threads,
 // no MUX2 register groups compiled in).
+// Diagonal table on the padded LDS tile: element e times entry e mod 2^k (its QSV2 tile index) of
+// the 2^k complex64 phases at tab; consecutive threads take consecutive elements, so both the LDS
+// accesses and the table reads (cached: every workgroup reads the same table) are contiguous.
+template <int K, int NT>
+__device__ __forceinline__ void diag_table(C2<float> *tile, const float *__restrict__ tab, int k, uint32_t tid) {
+    constexpr uint32_t NE = 1u << K;
+    const uint32_t km = (1u << k) - 1u;
+    const float2 *d = reinterpret_cast<const float2 *>(tab);
+#pragma unroll 4
+    for (uint32_t e = tid; e < NE; e += NT) {
+        const float2 w = d[e & km];
+        const C2<float> a = tile[lpad(e)];
+        tile[lpad(e)] = C2<float>{fmaf(a.x, w.x, -a.y * w.y), fmaf(a.x, w.y, a.y * w.x)};
+    }
+}
+
 template <typename T, int K, bool DN, bool D4 = false, int NT = kNT>
 __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                           const GroupDev *__restrict__ gd, const OpDev<T> *__restrict__ od,
@@ -1095,6 +1111,11 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
             __syncthreads();
             continue;
         } else if constexpr (DN && sizeof(T) == 4) {
+            if (G.type == 6) {
+                diag_table<K, NT>(reinterpret_cast<C2<float> *>(smem), wd + G.w_off, k, tid);
+                __syncthreads();
+                continue;
+            }
             if (G.type != 0) {
                 dense_group<K, NT / 64, false>(smem, G, wd, tid);
                 __syncthreads();

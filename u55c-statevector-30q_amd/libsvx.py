@@ -216,12 +216,13 @@ def run_batch(states, handles, passes, stream=None):
 
 def prepared_pass_stats(handle, pass_index):
     """Work of one pass after classification and fusion: MUX2 groups, MUX2 ops, dense units on 4
-    and 5 qubits, dense units on the exact FP32 path, pairs of 4-qubit units sharing an LDS round trip."""
+    and 5 qubits, dense units on the exact FP32 path, pairs of 4-qubit units sharing an LDS round trip,
+    diagonal tables (runs of diagonal ops applied as one table of phases)."""
     lib = load()
-    out = (ctypes.c_int32 * 6)()
+    out = (ctypes.c_int32 * 7)()
     _check(lib.svx_prepared_pass_stats(handle, pass_index, out))
     return dict(mux_groups=out[0], mux_ops=out[1], dense4=out[2], dense5=out[3], dense_exact=out[4],
-                dense_pairs=out[5])
+                dense_pairs=out[5], diag_tables=out[6])
 
 
 def permute(dst, src, perm, stream=None):
