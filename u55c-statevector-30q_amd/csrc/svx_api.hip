@@ -759,26 +759,65 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         // groups: maximal runs of ops whose targets fit R register bits; a dense op is a group of its own
         size_t i = 0;
         double log2_growth = 0;   // bound on the growth of the tile's largest |amplitude| over the pass
-        while (i < pops.size()) {
-            if (dense_of[i] >= 0 && dus[dense_of[i]].diag) {
-                // diagonal table: 2^k complex64 entries (8-byte aligned), |entries| bound the growth
-                const DenseUnit &du = dus[dense_of[i]];
-                GroupDev gd{};
-                gd.type = 6;
-                while (wmat.size() % 4) wmat.push_back(0.f);
-                gd.w_off = (int64_t)wmat.size();
-                double amax = 0;
-                for (size_t x = 0; 2 * x < du.u.size(); ++x) {
-                    wmat.push_back((float)du.u[2 * x]);
-                    wmat.push_back((float)du.u[2 * x + 1]);
+        // diagonal table: 2^k complex64 entries (16-byte aligned) appended to the operand array;
+        // returns their float offset; |entries| bound the growth
+        auto put_table = [&](const DenseUnit &du) {
+            while (wmat.size() % 4) wmat.push_back(0.f);
+            const int64_t off = (int64_t)wmat.size();
+            double amax = 0;
+            for (size_t x = 0; 2 * x < du.u.size(); ++x) {
+                wmat.push_back((float)du.u[2 * x]);
+                wmat.push_back((float)du.u[2 * x + 1]);
+                amax = std::max(amax, std::hypot(du.u[2 * x], du.u[2 * x + 1]));
+            }
+            log2_growth += std::log2(std::max(1.0, amax));
+            return off;
+        };
+        // the same table inside a MUX2 register group (register bits rb, in register order): entry
+        // r 2^(K-R) + s belongs to register r of slot s (element: s with zero bits inserted at the
+        // register bits, ascending, then register r's bits set), so the threads of a wavefront read
+        // consecutive entries for each register
+        auto put_table_regs = [&](const DenseUnit &du, const std::vector<int> &rb) {
+            while (wmat.size() % 4) wmat.push_back(0.f);
+            const int64_t off = (int64_t)wmat.size();
+            std::vector<int> bs(rb.begin(), rb.end());
+            std::sort(bs.begin(), bs.end());
+            const uint32_t nslot = 1u << (K - R), kmask = (1u << k) - 1u;
+            wmat.resize(wmat.size() + (size_t(2) << K));
+            double amax = 0;
+            for (uint32_t sl = 0; sl < nslot; ++sl) {
+                uint32_t base = sl;
+                for (int q = 0; q < R; ++q) base = ((base >> bs[q]) << (bs[q] + 1)) | (base & ((1u << bs[q]) - 1u));
+                for (uint32_t r = 0; r < (1u << R); ++r) {
+                    uint32_t e = base;
+                    for (int q = 0; q < R; ++q)
+                        if ((r >> q) & 1u) e |= 1u << rb[q];
+                    const size_t x = e & kmask, o = off + 2 * (size_t(r) * nslot + sl);
+                    wmat[o] = (float)du.u[2 * x];
+                    wmat[o + 1] = (float)du.u[2 * x + 1];
                     amax = std::max(amax, std::hypot(du.u[2 * x], du.u[2 * x + 1]));
                 }
-                log2_growth += std::log2(std::max(1.0, amax));
-                groups.push_back(gd);
-                ++i;
-                continue;
             }
-            if (dense_of[i] >= 0) {
+            log2_growth += std::log2(std::max(1.0, amax));
+            return off;
+        };
+        auto is_table = [&](size_t x) { return dense_of[x] >= 0 && dus[dense_of[x]].diag; };
+        while (i < pops.size()) {
+            if (is_table(i)) {
+                // a table with no MUX2 op before the next dense unit (or the end): its own group, the
+                // LDS tile swept in order (contiguous table reads); beside MUX2 ops it joins their group
+                size_t j = i;
+                while (j < pops.size() && is_table(j)) ++j;
+                if (j == pops.size() || dense_of[j] >= 0) {
+                    GroupDev gd{};
+                    gd.type = 6;
+                    gd.w_off = put_table(dus[dense_of[i]]);
+                    groups.push_back(gd);
+                    ++i;
+                    continue;
+                }
+            }
+            if (dense_of[i] >= 0 && !is_table(i)) {
                 const DenseUnit &du = dus[dense_of[i]];
                 const int f0 = du.f, F = std::max(4, f0);
                 GroupDev gd{};
@@ -882,6 +921,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             std::vector<int> tg;
             size_t j = i;
             for (; j < pops.size(); ++j) {
+                if (is_table(j)) continue;   // a table needs no register bits
                 if (dense_of[j] >= 0) break;
                 const int t = pops[j].t;
                 if (std::find(tg.begin(), tg.end(), t) == tg.end()) {
@@ -903,6 +943,13 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             }
             std::sort(gd.bs, gd.bs + R);
             for (size_t o = i; o < j; ++o) {
+                if (is_table(o)) {   // each register's amplitude times its table entry
+                    OpDev<T> od{};
+                    od.kind = K_TABLE;
+                    od.aux = (int32_t)put_table_regs(dus[dense_of[o]], rb);
+                    ops.push_back(od);
+                    continue;
+                }
                 OpDev<T> od = pops[o];
                 od.tr = (int)(std::find(rb.begin(), rb.end(), od.t) - rb.begin());
                 uint32_t cmr = 0;
@@ -995,8 +1042,11 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         for (int g = pd.group0; g < (int)groups.size(); ++g) {
             const GroupDev &G = groups[g];
             if (G.type == 0) {
+                int nt = 0;
+                for (int o = 0; o < G.nops; ++o) nt += ops[G.op0 + o].kind == K_TABLE;
                 lr.stats[0] += 1;
-                lr.stats[1] += G.nops;
+                lr.stats[1] += G.nops - nt;
+                lr.stats[6] += nt;
             } else if (G.type == 6) {
                 lr.stats[6] += 1;
             } else {

@@ -27,6 +27,8 @@ enum OpKind : int32_t {
     K_CXSWAP1 = 8, // controlled exact X, one control in the register group, none outside
     K_CPHASE1 = 9, // controlled diag(1, e), one control in the register group, none outside
     K_DENSE = 10,  // small-state kernel only: dense op, cmask = index of its GroupDev (type 2)
+    K_TABLE = 11,  // complex64 pass kernel: diagonal table, 2^k complex64 phases at float offset aux
+                   // of the dense operand array, indexed by the QSV2 tile index (no register bits)
 };
 
 struct PassDev {
@@ -50,8 +52,8 @@ struct GroupDev {
                                  // on a tile pre-scaled by the pass kernel (PassDev::tscale); 5: first of
                                  // a pair of 4-qubit type-4 units on disjoint bits applied in one LDS
                                  // round trip (the next group is the second; operands in pair order);
-                                 // 6: diagonal table, 2^k complex64 phases at w_off indexed by the QSV2
-                                 // tile index (a run of diagonal ops, complex64 single-buffer kernel)
+                                 // 6: diagonal table on its own (no MUX2 op beside it), 2^k complex64
+                                 // phases at w_off indexed by the QSV2 tile index (complex64 k_pass)
     int32_t f;                   // dense: qubits of the op (type 1: 4 or 5 after padding)
     int32_t dq[5];               // dense: workgroup-tile bit of the op's qubit i (matrix index bit i)
     int32_t dqs[5];              // dq sorted ascending (column deposit)
@@ -66,7 +68,7 @@ struct OpDev {
     int32_t kind, tr;            // kind, target register bit
     uint32_t cmr, cmt;           // control mask over register bits / over other tile bits
     int32_t t, cmask;            // raw tile-coordinate target and control mask (small kernel)
-    int32_t pad_[2];
+    int32_t aux, pad_;           // K_TABLE: float offset of the table
     T m0[8], m1[8];              // (re, im) of m00 m01 m10 m11
 };
 

@@ -1052,6 +1052,24 @@ __device__ __forceinline__ void diag_table(C2<float> *tile, const float *__restr
     }
 }
 
+// Diagonal table on a MUX2 register group: the host stores the phase of register r of slot s at
+// entry r NSLOT + s, so for each register the threads of a wavefront read consecutive entries.
+template <int R, uint32_t NSLOT>
+__device__ __forceinline__ void table_regs(C2<float> (&a)[1 << R], const float *__restrict__ tab, uint32_t s) {
+    const float2 *d = reinterpret_cast<const float2 *>(tab) + s;
+#pragma unroll
+    for (int r0 = 0; r0 < (1 << R); r0 += 8) {   // eight table reads in flight, few registers
+        float2 w[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) w[r] = d[(r0 + r) * NSLOT];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const C2<float> v = a[r0 + r];
+            a[r0 + r] = C2<float>{fmaf(v.x, w[r].x, -v.y * w[r].y), fmaf(v.x, w[r].y, v.y * w[r].x)};
+        }
+    }
+}
+
 template <typename T, int K, bool DN, bool D4 = false, int NT = kNT>
 __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                           const GroupDev *__restrict__ gd, const OpDev<T> *__restrict__ od,
@@ -1143,6 +1161,12 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
 #pragma unroll 1
             for (int o = 0; o < nops; ++o) {
                 const OpDev<T> &op = od[o0 + o];
+                if constexpr (DN && sizeof(T) == 4) {
+                    if (op.kind == K_TABLE) {
+                        table_regs<RR, NSLOT>(reinterpret_cast<C2<float>(&)[NR]>(a), wd + op.aux, s);
+                        continue;
+                    }
+                }
                 const bool ts = (base & op.cmt) == op.cmt;
                 apply_op<T, RR>(a, op, ts);
             }
