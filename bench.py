@@ -90,6 +90,8 @@ def cp_ladder_section(libsvx, circuit, compiler, torch, np, n=30, seed=300):
     stream = torch.cuda.current_stream()
     times = {}
     for name, s, hs in (("c64", s64, h64), ("c128", s128, h128)):
+        for h in hs:   # workgroup-to-tile order per pass by measurement (untimed)
+            libsvx.tune_prepared(s, h)
         libsvx.product_state(s, amps)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -251,7 +253,8 @@ def main():
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: an 8 GiB complex64 copy (svx_convert, calibration) twice, "
-                         "then one step of the 30-qubit workload; prints the kernel names")
+                         "the tile-order tuning launches (on the copy's destination), then one step of the "
+                         "30-qubit workload; prints the kernel names")
     a = ap.parse_args()
     if a.cpu_baseline_child:
         print(json.dumps(cpu_baseline()))
@@ -278,6 +281,8 @@ def main():
         for _ in range(2):
             libsvx.convert(dst, src)
         for h in hs:
+            libsvx.tune_prepared(dst, h)
+        for h in hs:
             libsvx.run_prepared(src, h)
         torch.cuda.synchronize()
         names = sorted({libsvx.prepared_kernel(h, p) for h in hs for p in range(libsvx.prepared_info(h)[1])})
@@ -293,6 +298,12 @@ def main():
     state = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
     libsvx.zero_state(state)
     stream = torch.cuda.current_stream()
+    # each pass's workgroup-to-tile order by measurement (untimed; bit-identical results), then |0..0>
+    for h in handles:
+        libsvx.tune_prepared(state, h)
+    libsvx.zero_state(state)
+    orders = [libsvx.prepared_pass_stats(h, p)["tile_order"] for h in handles
+              for p in range(libsvx.prepared_info(h)[1])]
 
     def step():
         for h in handles:
@@ -312,6 +323,8 @@ def main():
 
     # per-pass durations: each pass as its own prepared segment, HIP events on the launch stream
     single = [libsvx.prepare(seg, libsvx.C64, n) for _, w in prog for seg in compiler.split_passes(w)]
+    for h in single:
+        libsvx.tune_prepared(state, h)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in single]
     for (e0, e1), h in zip(evs, single):
         e0.record(stream)
@@ -370,7 +383,8 @@ def main():
         "config": {"workload": "u3cx_30q_d20_s1: 30 qubits, complex64, depth 20, seed 1, U3 + CX matching "
                                "(26..29 paired with 0..25)",
                    "n_qubits": n, "ops": st.n_ops, "passes": st.n_passes, "launches": n_launch,
-                   "tile_k": compiler.GPU_K, "compile_s": round(t_compile, 3)},
+                   "tile_k": compiler.GPU_K, "compile_s": round(t_compile, 3),
+                   "tile_order_tuned": {"xcd_contiguous": orders.count(-2), "identity": orders.count(0)}},
         "frac_of_peak": round(value / PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,

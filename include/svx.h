@@ -69,6 +69,16 @@ int svx_engine(void *st, int64_t ins_off, void *stream);
 int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle);
 /* Launch a prepared segment: kernels only, no allocation or copy (safe under stream capture). */
 int svx_run_prepared(void *state, int64_t handle, void *stream);
+
+/* Pick each pass's workgroup-to-tile order by measurement: every pass of the prepared segment that
+ * runs on the single-buffer or dense-only kernel over >= 64 workgroup tiles is launched 1 + reps
+ * times with each candidate order (each XCD a contiguous range of tiles, or the identity) on
+ * `state`, timed with HIP events (synchronously), and keeps the identity only when it is >= 3%
+ * faster.  The order changes which workgroup takes which tile, never the arithmetic, so results
+ * stay bit-identical.  `state` must be a state the segment may run on (its contents change: each
+ * tunable pass is applied 2 (1 + reps) times); not concurrently with runs of the same handle.  No
+ * reference counterpart (the FPGA engine's tile order is fixed by its reader). */
+int svx_tune_prepared(void *state, int64_t handle, int32_t reps, void *stream);
 /* Continuous batching: run pass passes[i] of prepared segment handles[i] on states[i] (complex64,
  * i < n_items <= 4096) with one launch per kernel shape, the items' tiles side by side in one grid.
  * Each state gets the same arithmetic as in svx_run_prepared, so results are bit-identical alone and
@@ -87,8 +97,9 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
 /* Work of pass `pass` of a prepared segment after classification and pass fusion:
  * out[0] MUX2 register groups, out[1] MUX2 ops, out[2] dense units on 4 qubits, out[3] dense units on
  * 5 qubits, out[4] dense units on the exact FP32 matrix-core path, out[5] pairs of 4-qubit units applied
- * in one LDS round trip, out[6] diagonal tables (runs of diagonal ops applied as one table of phases)
- * (out has 7 entries).  No reference counterpart. */
+ * in one LDS round trip, out[6] diagonal tables (runs of diagonal ops applied as one table of phases),
+ * out[7] the workgroup-to-tile order of the pass (-2: each XCD takes a contiguous range of tiles,
+ * 0: identity; see svx_tune_prepared) (out has 8 entries).  No reference counterpart. */
 int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out);
 
 /* Qubit permutation perm[new_bit] = old_bit of a 2^n state, out of place (dst != src), bit-exact.

@@ -441,3 +441,44 @@ def test_dense_only_kernel_large_states(gpu, n, k, tile_scale, monkeypatch):
     assert not any(nm.startswith("k_pass_d4<") for nm in names["off"]), names
     assert np.array_equal(got["auto"], got["off"])
     assert abs(np.linalg.norm(got["auto"].astype(np.complex128)) - 1) < 1e-5
+
+
+@pytest.mark.parametrize("dt", ["c64", "c128"])
+def test_tile_order_tuning_keeps_results(gpu, dt, monkeypatch):
+    """svx_tune_prepared picks each pass's workgroup-to-tile order by timing; the order only changes
+    which workgroup takes which tile, so the tuned run, the default order and the identity order
+    (SVX_TILE_ORDER=0) give bit-identical states, and the stats report the order in use."""
+    from u55c_statevector_30q_amd import libsvx
+    torch = _torch()
+    n, k = 22, 13
+    rng = np.random.default_rng(77 if dt == "c64" else 78)
+    seg = random_segment(rng, n, k, n_passes=4, max_ops=20, max_m=7)
+    npdt = np.complex64 if dt == "c64" else np.complex128
+    code = libsvx.C64 if dt == "c64" else libsvx.C128
+    s0 = _rand_state(rng, n, npdt)
+    ref = model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128)
+
+    def run(h):
+        t = torch.from_numpy(s0.copy()).to("cuda")
+        libsvx.run_prepared(t, h)
+        torch.cuda.synchronize()
+        return t.cpu().numpy()
+
+    h = libsvx.prepare(seg, code, n)
+    before = run(h)
+    scratch = torch.from_numpy(s0.copy()).to("cuda")
+    libsvx.tune_prepared(scratch, h, reps=1)
+    orders = [libsvx.prepared_pass_stats(h, p)["tile_order"] for p in range(libsvx.prepared_info(h)[1])]
+    assert set(orders) <= {-2, 0}, orders
+    tuned = run(h)
+    monkeypatch.setenv("SVX_TILE_ORDER", "0")
+    hi = libsvx.prepare(seg, code, n)
+    assert all(libsvx.prepared_pass_stats(hi, p)["tile_order"] == 0 for p in range(libsvx.prepared_info(hi)[1]))
+    ident = run(hi)
+    assert np.array_equal(before, tuned) and np.array_equal(before, ident)
+    if dt == "c64":
+        _check_c64(ref, before, (n, k))
+    else:
+        assert np.max(np.abs(before - ref)) < 1e-12
+    with pytest.raises(ValueError):
+        libsvx.tune_prepared(scratch, h, reps=0)

@@ -240,6 +240,7 @@ struct LaunchInfo {
     bool empty;           // no op changes the state (skip launch)
     bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
+    bool tunable;         // single-buffer / dense-only launch whose tile order svx_tune_prepared may switch
     int32_t stats[7];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32), pairs,
                           // diagonal tables
 };
@@ -248,9 +249,11 @@ struct Prepared {
     int dtype = 0, nbits = 0;
     std::vector<int32_t> words;
     std::string pick;           // SVX_PASS_KERNEL at preparation time
-    void *dev = nullptr;        // [PassDev x P][GroupDev x G][OpDev<T> x O]
-    size_t pass_off = 0, group_off = 0, op_off = 0, w_off = 0;
+    void *dev = nullptr;        // [PassDev x P][PassDev x P, identity tile order][GroupDev x G][OpDev<T> x O][operands]
+    size_t pass_off = 0, alt_off = 0, group_off = 0, op_off = 0, w_off = 0;
     std::vector<LaunchInfo> launches;
+    std::vector<int8_t> alt;    // per pass: 1 = launch with the identity tile order (set by svx_tune_prepared)
+    int torder = -2;            // default workgroup -> tile order of the passes (PassDev::torder)
     int device = 0;
 };
 
@@ -717,6 +720,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         li.pass_index = (int64_t)passes.size();
         li.empty = pd.nops == 0;
         li.ntiles = uint64_t(1) << (nbits - K);
+        // the tile order matters for HBM access once many workgroups run at a time; an explicit
+        // SVX_TILE_ORDER fixes it
+        li.tunable = mode == 1 && !oenv && li.ntiles >= 64;
         if (mode == 0) {
             li.grid = 1;
             li.lds = 0;
@@ -1059,13 +1065,20 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     }
     // layout: passes, groups, ops (256-byte aligned sections)
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    // the same passes with the identity workgroup -> tile order, for svx_tune_prepared's choice
+    std::vector<PassDev> alt = passes;
+    for (PassDev &a : alt) a.torder = 0;
     pr->pass_off = 0;
-    pr->group_off = al(passes.size() * sizeof(PassDev));
+    pr->alt_off = al(passes.size() * sizeof(PassDev));
+    pr->group_off = pr->alt_off + al(alt.size() * sizeof(PassDev));
     pr->op_off = pr->group_off + al(groups.size() * sizeof(GroupDev));
     pr->w_off = pr->op_off + al(ops.size() * sizeof(OpDev<T>));
     const size_t total = std::max<size_t>(256, pr->w_off + al(wmat.size() * sizeof(float)));
     std::vector<unsigned char> host(total, 0);
     if (!passes.empty()) std::memcpy(host.data() + pr->pass_off, passes.data(), passes.size() * sizeof(PassDev));
+    if (!alt.empty()) std::memcpy(host.data() + pr->alt_off, alt.data(), alt.size() * sizeof(PassDev));
+    pr->alt.assign(passes.size(), 0);
+    pr->torder = torder;
     if (!groups.empty()) std::memcpy(host.data() + pr->group_off, groups.data(), groups.size() * sizeof(GroupDev));
     if (!ops.empty()) std::memcpy(host.data() + pr->op_off, ops.data(), ops.size() * sizeof(OpDev<T>));
     if (!wmat.empty()) std::memcpy(host.data() + pr->w_off, wmat.data(), wmat.size() * sizeof(float));
@@ -1130,34 +1143,36 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     return 0;
 }
 
+// one pass launch of a prepared segment (default or tuned tile order)
+hipError_t launch_one(void *state, const Prepared *pr, const LaunchInfo &li, hipStream_t s) {
+    unsigned char *base = static_cast<unsigned char *>(pr->dev);
+    const PassDev *pd = reinterpret_cast<const PassDev *>(base + (pr->alt[li.pass_index] ? pr->alt_off : pr->pass_off)) +
+                        li.pass_index;
+    const GroupDev *gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
+    const float *wd = li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr;
+    if (li.d4)
+        return launch_pass_d4(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
+                              li.grid, li.lds, s);
+    if (li.pers)
+        return launch_pass_pers(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
+                                li.ntiles, li.grid, li.lds, s);
+    if (li.dma)
+        return launch_pass_dma(li.K + li.dma_variant, state, pd, gd,
+                               reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd, li.ntiles, li.grid,
+                               li.lds, s);
+    if (pr->dtype == SVX_C64)
+        return launch_pass_c64(li.K, li.small, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
+                               wd, li.grid, li.lds, s);
+    return launch_pass_c128(li.K, li.small, state, pd, gd, reinterpret_cast<const OpDev<double> *>(base + pr->op_off),
+                            li.grid, li.lds, s);
+}
+
 int run_prepared(void *state, Prepared *pr, hipStream_t s) {
     if (!state) return fail(SVX_E_ARG, "null state pointer");
     if (reinterpret_cast<uintptr_t>(state) & 15) return fail(SVX_E_ARG, "state must be 16-byte aligned");
-    unsigned char *base = static_cast<unsigned char *>(pr->dev);
-    const PassDev *pd = reinterpret_cast<const PassDev *>(base + pr->pass_off);
-    const GroupDev *gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
     for (const LaunchInfo &li : pr->launches) {
         if (li.empty) continue;
-        hipError_t e;
-        if (li.d4)
-            e = launch_pass_d4(li.K, state, pd + li.pass_index, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
-                               li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr, li.grid, li.lds, s);
-        else if (li.pers)
-            e = launch_pass_pers(li.K, state, pd + li.pass_index, gd,
-                                 reinterpret_cast<const OpDev<float> *>(base + pr->op_off), li.ntiles, li.grid, li.lds, s);
-        else if (li.dma)
-            e = launch_pass_dma(li.K + li.dma_variant, state, pd + li.pass_index, gd,
-                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
-                                li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr,
-                                li.ntiles, li.grid, li.lds, s);
-        else if (pr->dtype == SVX_C64)
-            e = launch_pass_c64(li.K, li.small, state, pd + li.pass_index, gd,
-                                reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
-                                li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr,
-                                li.grid, li.lds, s);
-        else
-            e = launch_pass_c128(li.K, li.small, state, pd + li.pass_index, gd,
-                                 reinterpret_cast<const OpDev<double> *>(base + pr->op_off), li.grid, li.lds, s);
+        const hipError_t e = launch_one(state, pr, li, s);
         if (e != hipSuccess) return hipfail(e, "pass kernel launch");
     }
     return 0;
@@ -1275,6 +1290,47 @@ int svx_run_prepared(void *state, int64_t handle, void *stream) {
     return run_prepared(state, pr, (hipStream_t)stream);
 }
 
+int svx_tune_prepared(void *state, int64_t handle, int32_t reps, void *stream) {
+    Prepared *pr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_prep_mu);
+        if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
+            return fail(SVX_E_ARG, "unknown prepared-segment handle");
+        pr = g_prepared[handle - 1].get();
+    }
+    if (!state) return fail(SVX_E_ARG, "null state pointer");
+    if (reinterpret_cast<uintptr_t>(state) & 15) return fail(SVX_E_ARG, "state must be 16-byte aligned");
+    if (reps < 1 || reps > 64) return fail(SVX_E_ARG, "reps must be in 1..64");
+    const hipStream_t s = (hipStream_t)stream;
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        hipEventDestroy(ev[0]);
+        return hipfail(hipErrorOutOfMemory, "event");
+    }
+    int rc = 0;
+    for (const LaunchInfo &li : pr->launches) {
+        if (li.empty || !li.tunable) continue;
+        float ms[2] = {0.f, 0.f};
+        for (int c = 0; c < 2 && !rc; ++c) {
+            pr->alt[li.pass_index] = (int8_t)c;
+            hipError_t e = launch_one(state, pr, li, s);   // untimed: first launch of this order
+            if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_one(state, pr, li, s);
+            if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms[c], ev[0], ev[1]);
+            if (e != hipSuccess) rc = hipfail(e, "tile-order timing");
+        }
+        // the identity order only when clearly faster (timing noise stays on the default)
+        pr->alt[li.pass_index] = (!rc && ms[1] < 0.97f * ms[0]) ? 1 : 0;
+        if (rc) break;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    return rc;
+}
+
 int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *passes, int32_t n_items,
                   void *stream) {
     if (n_items < 0 || (n_items > 0 && (!states || !handles || !passes))) return fail(SVX_E_ARG, "bad batch arrays");
@@ -1386,6 +1442,7 @@ int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out) {
     Prepared *pr = g_prepared[handle - 1].get();
     if (pass < 0 || pass >= (int32_t)pr->launches.size() || !out) return fail(SVX_E_ARG, "bad pass index");
     for (int i = 0; i < 7; ++i) out[i] = pr->launches[pass].stats[i];
+    out[7] = pr->alt[pr->launches[pass].pass_index] ? 0 : pr->torder;
     return 0;
 }
 

@@ -19,7 +19,7 @@ C64, C128 = 0, 1
 SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -1, -2, -3, -4, -5, -6
 
 SYMBOLS = ["svx_init", "svx_run_segment", "svx_engine", "svx_prepare", "svx_run_prepared",
-           "svx_prepared_info", "svx_prepared_kernel", "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
+           "svx_tune_prepared", "svx_prepared_info", "svx_prepared_kernel", "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
            "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync", "svx_last_error",
            "svx_version"]
 
@@ -43,6 +43,7 @@ _SIGS = {
     "svx_engine": ([_VP, _I64, _VP], _I32),
     "svx_prepare": ([_PI32, _I64, _I32, _I32, ctypes.POINTER(_I64)], _I32),
     "svx_run_prepared": ([_VP, _I64, _VP], _I32),
+    "svx_tune_prepared": ([_VP, _I64, _I32, _VP], _I32),
     "svx_prepared_info": ([_I64, _PI32, _PI32], _I32),
     "svx_prepared_kernel": ([_I64, _I32, ctypes.c_char_p, _I32], _I32),
     "svx_prepared_pass_stats": ([_I64, _I32, _PI32, ], _I32),
@@ -184,6 +185,14 @@ def run_prepared(state, handle, stream=None):
     _check(lib.svx_run_prepared(_dev(state), handle, _stream(stream)))
 
 
+def tune_prepared(state, handle, reps=2, stream=None):
+    """svx_tune_prepared: pick each pass's workgroup-to-tile order by timing both candidates on
+    `state` (synchronous; the passes are applied 2 (1 + reps) times to it, so use a state whose
+    contents may change).  Results of later runs stay bit-identical."""
+    lib = load()
+    _check(lib.svx_tune_prepared(_dev(state), handle, int(reps), _stream(stream)))
+
+
 def prepared_info(handle):
     lib = load()
     a, b = ctypes.c_int32(0), ctypes.c_int32(0)
@@ -217,12 +226,13 @@ def run_batch(states, handles, passes, stream=None):
 def prepared_pass_stats(handle, pass_index):
     """Work of one pass after classification and fusion: MUX2 groups, MUX2 ops, dense units on 4
     and 5 qubits, dense units on the exact FP32 path, pairs of 4-qubit units sharing an LDS round trip,
-    diagonal tables (runs of diagonal ops applied as one table of phases)."""
+    diagonal tables (runs of diagonal ops applied as one table of phases), and the workgroup-to-tile
+    order the pass runs with (-2: XCD-contiguous, 0: identity)."""
     lib = load()
-    out = (ctypes.c_int32 * 7)()
+    out = (ctypes.c_int32 * 8)()
     _check(lib.svx_prepared_pass_stats(handle, pass_index, out))
     return dict(mux_groups=out[0], mux_ops=out[1], dense4=out[2], dense5=out[3], dense_exact=out[4],
-                dense_pairs=out[5], diag_tables=out[6])
+                dense_pairs=out[5], diag_tables=out[6], tile_order=out[7])
 
 
 def permute(dst, src, perm, stream=None):
