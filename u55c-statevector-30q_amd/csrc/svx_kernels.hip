@@ -1711,20 +1711,34 @@ __global__ __launch_bounds__(256) void k_permute(C2<T> *__restrict__ dst, const 
     if (sizeof(T) == 4 && c >= 1) {
         // complex64: adjacent element pairs (16 bytes) on both sides; LDS pads 2 elements per 32 so
         // new-order pairs stay 16-byte aligned
+        // every 16-byte load of the tile (at most 8 per thread: v <= 12) is issued before the first
+        // LDS write, and both sides stream past the caches (each amplitude is moved once)
         auto lp = [](uint32_t e) { return e + ((e >> 5) << 1); };
         const uint32_t b0 = uint32_t(pd->rdlo[1]);    // LDS position bit of old bit 0
-        for (uint32_t q = 2 * threadIdx.x; q < NE; q += 512) {
-            const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
-            const float4 v2 = *reinterpret_cast<const float4 *>(
-                reinterpret_cast<const float *>(src) + 2 * (ob | (q & cm) | pd->oldhi[q >> c]));
-            tile[lp(pos)] = C2<T>{T(v2.x), T(v2.y)};
-            tile[lp(pos | b0)] = C2<T>{T(v2.z), T(v2.w)};
+        float4 r[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t q = 2 * threadIdx.x + 512u * i;
+            if (q < NE) r[i] = ld_stream(reinterpret_cast<const float *>(src) + 2 * (ob | (q & cm) | pd->oldhi[q >> c]));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t q = 2 * threadIdx.x + 512u * i;
+            if (q < NE) {
+                const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
+                tile[lp(pos)] = C2<T>{T(r[i].x), T(r[i].y)};
+                tile[lp(pos | b0)] = C2<T>{T(r[i].z), T(r[i].w)};
+            }
         }
         __syncthreads();
-        for (uint32_t l = 2 * threadIdx.x; l < NE; l += 512) {
-            const C2<T> a0 = tile[lp(l)], a1 = tile[lp(l) + 1];
-            *reinterpret_cast<float4 *>(reinterpret_cast<float *>(dst) + 2 * (nb | (l & cm) | pd->newhi[l >> c])) =
-                make_float4(float(a0.x), float(a0.y), float(a1.x), float(a1.y));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t l = 2 * threadIdx.x + 512u * i;
+            if (l < NE) {
+                const C2<T> a0 = tile[lp(l)], a1 = tile[lp(l) + 1];
+                st_stream(reinterpret_cast<float *>(dst) + 2 * (nb | (l & cm) | pd->newhi[l >> c]),
+                          make_float4(float(a0.x), float(a0.y), float(a1.x), float(a1.y)));
+            }
         }
         return;
     }
