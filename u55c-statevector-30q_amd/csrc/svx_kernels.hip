@@ -1033,9 +1033,6 @@ __device__ __forceinline__ void tile_out_swz(C2<float> *__restrict__ st, const u
     }
 }
 
-// One workgroup tile of one pass: `tile` selects the 2^(K-k) QSV2 tiles the workgroup owns.
-// DN: the pass has dense groups; D4: dense-only instance (every group a dense 4-qubit unit, NT threads,
-// no MUX2 register groups compiled in).
 // Diagonal table on the padded LDS tile: element e times entry e mod 2^k (its QSV2 tile index) of
 // the 2^k complex64 phases at tab; consecutive threads take consecutive elements, so both the LDS
 // accesses and the table reads (cached: every workgroup reads the same table) are contiguous.
@@ -1070,6 +1067,9 @@ __device__ __forceinline__ void table_regs(C2<float> (&a)[1 << R], const float *
     }
 }
 
+// One workgroup tile of one pass: `tile` selects the 2^(K-k) QSV2 tiles the workgroup owns.
+// DN: the pass has dense groups or diagonal tables; D4: dense-only instance (every group a dense
+// 4-qubit unit, NT threads, no MUX2 register groups compiled in).
 template <typename T, int K, bool DN, bool D4 = false, int NT = kNT>
 __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev *__restrict__ pd,
                                           const GroupDev *__restrict__ gd, const OpDev<T> *__restrict__ od,
