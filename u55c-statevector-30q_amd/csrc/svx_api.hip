@@ -1180,7 +1180,7 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
 
 // ------------------------------------------------------------------------------------------ permute
 std::mutex g_perm_mu;
-std::map<std::vector<int32_t>, void *> g_perm_cache;   // key: [device, n, perm...] -> PermDev
+std::map<std::vector<int32_t>, std::pair<void *, uint32_t>> g_perm_cache;   // [device, n, perm...] -> (PermDev, grid)
 
 int build_perm(int n, const int32_t *perm, PermDev *pd, uint32_t *grid) {
     std::vector<int> inv(n, -1);
@@ -1521,16 +1521,17 @@ int svx_permute(void *dst, const void *src, int32_t n, const int32_t *perm, int3
     uint32_t grid = 0;
     {
         std::lock_guard<std::mutex> lk(g_perm_mu);
-        PermDev pd;
-        rc = build_perm(n, perm, &pd, &grid);
-        if (rc) return rc;
         auto it = g_perm_cache.find(key);
         if (it != g_perm_cache.end()) {
-            pdv = it->second;
+            pdv = it->second.first;
+            grid = it->second.second;
         } else {
+            PermDev pd;
+            rc = build_perm(n, perm, &pd, &grid);
+            if (rc) return rc;
             HIPCHK(hipMalloc(&pdv, sizeof(PermDev)));
             HIPCHK(hipMemcpy(pdv, &pd, sizeof(PermDev), hipMemcpyHostToDevice));
-            g_perm_cache[key] = pdv;
+            g_perm_cache[key] = {pdv, grid};
         }
     }
     const int v = n <= 12 ? n : 12;
