@@ -251,6 +251,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix section")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-baseline-child", action="store_true")
+    ap.add_argument("--ladder-probe", action="store_true",
+                    help="for rocprofv3 kernel traces: the 30-qubit controlled-phase ladder's complex64 passes "
+                         "(tile orders tuned first), once, then the bit reversal twice; prints per-pass work")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: an 8 GiB complex64 copy (svx_convert, calibration) twice, "
                          "the tile-order tuning launches (on the copy's destination), then one step of the "
@@ -272,6 +275,26 @@ def main():
     caps = libsvx.init(0)
     n = 30
     qc = circuit.workload_gpu30()
+    if a.ladder_probe:
+        lq = circuit.cp_ladder(n)
+        prog, fmap, st = compiler.compile_gpu(lq)
+        single = [seg for _, w in prog for seg in compiler.split_passes(w)]
+        hs = [libsvx.prepare(w, libsvx.C64, n) for w in single]
+        s64 = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
+        t64 = torch.empty_like(s64)
+        libsvx.zero_state(s64)
+        for h in hs:
+            libsvx.tune_prepared(s64, h)
+        th, ph = circuit.product_state_angles(n, 300)
+        libsvx.product_state(s64, np.stack([np.cos(th / 2), np.exp(1j * ph) * np.sin(th / 2)], axis=1))
+        for h in hs:
+            libsvx.run_prepared(s64, h)
+        for _ in range(2):
+            libsvx.permute(t64, s64, list(range(n))[::-1])
+        torch.cuda.synchronize()
+        work = [dict(libsvx.prepared_pass_stats(h, 0), kernel=libsvx.prepared_kernel(h, 0)) for h in hs]
+        print(json.dumps({"passes": len(hs), "per_pass": work}))
+        return
     if a.traffic_probe:
         prog, fmap, st = compiler.compile_gpu(qc)
         hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
