@@ -137,3 +137,33 @@ def test_cp_ladder_tables_and_fewer_mux_ops(gpu):
     got = model.unpermute(s.cpu().numpy(), n, fmap)
     assert tables >= 5 and mux < sum(g.name == "cp" for g in qc.data) // 2, (tables, mux)
     _check(gold, got, "cp ladder 20")
+
+
+def test_diagonal_tables_in_batched_launches(gpu):
+    """Batched launches (svx_run_batch: one pass of several states per launch, the serving path)
+    run segments with diagonal tables, alone and in the batch, with bit-identical results."""
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    n, k = 16, 13
+    items = []
+    for seed in range(3):
+        rng = np.random.default_rng(9100 + seed)
+        seg = _segment(rng, n, k, n_passes=3, nops=40, p_other=0.2, max_m=4)
+        s0 = _rand_state(rng, n).astype(np.complex64)
+        h = libsvx.prepare(seg, libsvx.C64, n)
+        assert sum(libsvx.prepared_pass_stats(h, p)["diag_tables"] for p in range(3)) >= 1
+        items.append((h, s0, seg))
+    alone = []
+    for h, s0, _ in items:
+        t = torch.from_numpy(s0.copy()).to("cuda")
+        libsvx.run_prepared(t, h)
+        torch.cuda.synchronize()
+        alone.append(t.cpu().numpy())
+    states = [torch.from_numpy(s0.copy()).to("cuda") for _, s0, _ in items]
+    for p in range(3):
+        libsvx.run_batch(states, [h for h, _, _ in items], [p] * len(items))
+    torch.cuda.synchronize()
+    for (h, s0, seg), st, a in zip(items, states, alone):
+        got = st.cpu().numpy()
+        assert np.array_equal(got, a)
+        _check(model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128), got, "batched tables")
