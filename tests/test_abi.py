@@ -36,3 +36,13 @@ def test_errors_map_without_a_device():
     rc = lib.svx_init(0, None)
     assert rc == libsvx.SVX_E_NODEV
     assert lib.svx_last_error()
+
+
+def test_bench_is_one_gpu_only():
+    """bench.py measures one GPU: any other --gpus exits with status 2 and says so, before touching a
+    device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr, (r.returncode, r.stderr)
