@@ -343,8 +343,9 @@ void cmatmul_left(const std::vector<double> &b, std::vector<double> &a, int D) {
 // when their target/control bits intersect) into blocks of at most cfg.width tile bits, each block
 // grown by the ready op that adds the fewest new bits (earliest on ties); a block whose MUX2 cost
 // exceeds the cost of one dense op becomes one dense unit (product of its ops in FP64), the others
-// keep their ops.  Only ops on disjoint bits change order, so the result equals the pass's product
-// of ops up to rounding.
+// keep their ops.  A small beam search over the block sequence (other seed ops for each block) can
+// replace that greedy partition when it needs fewer dense units.  Only ops on disjoint bits change
+// order, so the result equals the pass's product of ops up to rounding.
 template <typename T>
 void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vector<DenseUnit> &dus,
                const FuseCfg &cfg) {
@@ -365,6 +366,7 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
     }
     std::vector<int> npred(n, 0);
     std::vector<std::vector<int>> succ(n);
+    std::vector<uint64_t> pm(n, 0);   // predecessors as a bit mask (passes of at most 64 ops)
     int last[32];
     for (int b = 0; b < 32; ++b) last[b] = -1;
     for (int i = 0; i < n; ++i) {
@@ -375,46 +377,138 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
                 last[b] = i;
             }
         npred[i] = (int)deps.size();
-        for (int d : deps) succ[d].push_back(i);
+        for (int d : deps) {
+            succ[d].push_back(i);
+            if (d < 64) pm[i] |= uint64_t(1) << d;
+        }
     }
-    std::set<int> ready;
-    for (int i = 0; i < n; ++i)
-        if (npred[i] == 0) ready.insert(i);
-    std::vector<OpDev<T>> out;
-    std::vector<int> out_dense;
-    while (!ready.empty()) {
-        uint32_t S = 0;
+    struct Block {
         std::vector<int> mem;
-        bool single = false;
-        while (true) {
-            int best = -1, bestadd = 99;
-            for (int i : ready) {
-                if (!fusable[i]) continue;
-                const int add = __builtin_popcount(qm[i] & ~S);
-                if (__builtin_popcount(S) + add > cfg.width) continue;
-                if (add < bestadd) {
-                    best = i;
-                    bestadd = add;
+        bool single = false;   // a non-fusable op (wide controls, a dense record or table) alone
+    };
+    // greedy partition: each block grown from the ready ops by the one adding the fewest new bits
+    std::vector<Block> greedy;
+    {
+        std::vector<int> np = npred;
+        std::set<int> ready;
+        for (int i = 0; i < n; ++i)
+            if (np[i] == 0) ready.insert(i);
+        while (!ready.empty()) {
+            uint32_t S = 0;
+            Block blk;
+            while (true) {
+                int best = -1, bestadd = 99;
+                for (int i : ready) {
+                    if (!fusable[i]) continue;
+                    const int add = __builtin_popcount(qm[i] & ~S);
+                    if (__builtin_popcount(S) + add > cfg.width) continue;
+                    if (add < bestadd) {
+                        best = i;
+                        bestadd = add;
+                    }
+                }
+                if (best < 0) {
+                    if (!blk.mem.empty()) break;
+                    best = *ready.begin();
+                    blk.single = true;
+                }
+                ready.erase(best);
+                blk.mem.push_back(best);
+                S |= qm[best];
+                for (int s2 : succ[best])
+                    if (--np[s2] == 0) ready.insert(s2);
+                if (blk.single) break;
+            }
+            greedy.push_back(std::move(blk));
+        }
+    }
+    auto is_dense = [&](const Block &b) {
+        if (b.single) return false;
+        double mux = 0;
+        uint32_t S = 0;
+        for (int i : b.mem) {
+            mux += kOpCost[pops[i].kind];
+            S |= qm[i];
+        }
+        const double dcost = std::max(4, __builtin_popcount(S)) == 5 ? cfg.cost5 : cfg.cost4;
+        return cfg.all_dense || (b.mem.size() >= 2 && mux > dcost);
+    };
+    auto score = [&](const std::vector<Block> &bl, int &units, int &left) {
+        units = left = 0;
+        for (const Block &b : bl) {
+            if (is_dense(b)) ++units;
+            else left += (int)b.mem.size();
+        }
+    };
+    std::vector<Block> chosen = std::move(greedy);
+    // beam search over block sequences (each block seeded by one ready op and grown like the greedy
+    // one), keeping the 8 partial partitions that have placed the most ops; its partition replaces
+    // the greedy one when it needs fewer dense units and leaves no more MUX2 ops
+    if (n <= 64 && !cfg.all_dense) {
+        struct St {
+            uint64_t done;
+            std::vector<Block> path;
+        };
+        const uint64_t full = n == 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
+        std::vector<St> cur(1, St{0, {}});
+        std::vector<Block> found;
+        for (int depth = 0; depth < n && found.empty() && !cur.empty(); ++depth) {
+            std::vector<St> nxt;
+            std::set<uint64_t> seen;
+            for (const St &st : cur) {
+                for (int r = 0; r < n; ++r) {
+                    if (((st.done >> r) & 1u) || (pm[r] & ~st.done)) continue;
+                    uint64_t d = st.done | (uint64_t(1) << r);
+                    Block b;
+                    b.mem.push_back(r);
+                    if (!fusable[r]) {
+                        b.single = true;
+                    } else {
+                        uint32_t S = qm[r];
+                        while (true) {
+                            int best = -1, bestadd = 99;
+                            for (int i = 0; i < n; ++i) {
+                                if (((d >> i) & 1u) || !fusable[i] || (pm[i] & ~d)) continue;
+                                const int add = __builtin_popcount(qm[i] & ~S);
+                                if (__builtin_popcount(S) + add > cfg.width) continue;
+                                if (add < bestadd) {
+                                    best = i;
+                                    bestadd = add;
+                                }
+                            }
+                            if (best < 0) break;
+                            d |= uint64_t(1) << best;
+                            b.mem.push_back(best);
+                            S |= qm[best];
+                        }
+                    }
+                    if (!seen.insert(d).second) continue;
+                    St ns{d, st.path};
+                    ns.path.push_back(std::move(b));
+                    if (d == full && found.empty()) found = ns.path;
+                    nxt.push_back(std::move(ns));
                 }
             }
-            if (best < 0) {
-                if (!mem.empty()) break;
-                best = *ready.begin();   // a non-fusable op (wide controls or a dense record) alone
-                single = true;
-            }
-            ready.erase(best);
-            mem.push_back(best);
-            S |= qm[best];
-            for (int s2 : succ[best])
-                if (--npred[s2] == 0) ready.insert(s2);
-            if (single) break;
+            std::stable_sort(nxt.begin(), nxt.end(), [](const St &a, const St &b) {
+                return __builtin_popcountll(a.done) > __builtin_popcountll(b.done);
+            });
+            if (nxt.size() > 8) nxt.resize(8);
+            cur.swap(nxt);
         }
-        double mux = 0;
-        if (!single)
-            for (int i : mem) mux += kOpCost[pops[i].kind];
-        const int nq = __builtin_popcount(S);
-        const double dcost = std::max(4, nq) == 5 ? cfg.cost5 : cfg.cost4;
-        if (!single && (cfg.all_dense || (mem.size() >= 2 && mux > dcost))) {
+        if (!found.empty()) {
+            int gu, gl, bu, bl;
+            score(chosen, gu, gl);
+            score(found, bu, bl);
+            if (bu < gu && bl <= gl) chosen = std::move(found);
+        }
+    }
+    std::vector<OpDev<T>> out;
+    std::vector<int> out_dense;
+    for (const Block &blk : chosen) {
+        if (is_dense(blk)) {
+            uint32_t S = 0;
+            for (int i : blk.mem) S |= qm[i];
+            const int nq = __builtin_popcount(S);
             std::vector<int> qs;
             for (int b = 0; b < 32; ++b)
                 if ((S >> b) & 1u) qs.push_back(b);
@@ -423,9 +517,9 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
             du.f = nq;
             for (int q = 0; q < nq; ++q) du.q[q] = qs[q];
             du.u.assign(size_t(2) * D * D, 0.0);
-            for (int d = 0; d < D; ++d) du.u[2 * (size_t(d) * D + d)] = 1.0;
+            for (int dd = 0; dd < D; ++dd) du.u[2 * (size_t(dd) * D + dd)] = 1.0;
             std::vector<double> m;
-            for (int i : mem) {
+            for (int i : blk.mem) {
                 mux_matrix(pops[i], qs, m);
                 cmatmul_left(m, du.u, D);
             }
@@ -436,7 +530,7 @@ void fuse_pass(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
             out_dense.push_back((int)dus.size());
             dus.push_back(std::move(du));
         } else {
-            for (int i : mem) {
+            for (int i : blk.mem) {
                 out.push_back(pops[i]);
                 out_dense.push_back(dense_of[i]);
             }
