@@ -33,6 +33,43 @@ extern "C" {
 #define SVX_E_HIP (-5)      /* HIP runtime error (RuntimeError) */
 #define SVX_E_NODEV (-6)    /* no usable gfx950 device (RuntimeError) */
 
+/* How a segment is prepared.  Every configuration is per call (no process-wide state): two
+ * threads preparing the same stream with different options get separate handles.
+ * svx_default_options() fills the defaults; `size` must be sizeof(svx_options). */
+#define SVX_FUSE_AUTO 0      /* cluster a pass's MUX2 ops into dense units where the units are cheaper */
+#define SVX_FUSE_OFF 1       /* every op on the MUX2 path */
+#define SVX_FUSE_DENSE4 2    /* every block of <= 4 tile bits a dense 4-qubit unit */
+#define SVX_DENSE_F32 0      /* dense units on FP32 matrix cores (v_mfma_f32_32x32x2f32): default */
+#define SVX_DENSE_SPLIT16 1  /* opt-in: split-FP16 matrix cores (hi 11 bits + lo FP16, lo x lo dropped,
+                                block-scaled below 2^15): ~2^-21 error per product relative to the
+                                tile/column maximum, so an amplitude 1e-9 below its tile maximum keeps
+                                only ~3 significant digits; faster than FP32 matrix cores */
+#define SVX_KERNEL_AUTO 0    /* per-pass choice (single-buffer / dense-only kernels) */
+#define SVX_KERNEL_SINGLE 1  /* always the single-buffer pass kernel */
+#define SVX_KERNEL_DMA 2     /* double-buffered pass kernel (loader wave + workers), complex64, k in {12,13} */
+#define SVX_ORDER_TUNABLE 0  /* XCD-contiguous tile order; svx_tune_prepared may switch passes to identity */
+#define SVX_ORDER_IDENTITY 1 /* fixed identity workgroup -> tile order */
+#define SVX_ORDER_XCD 2      /* fixed XCD-contiguous order (not tunable) */
+#define SVX_MUX_AUTO 0       /* uncontrolled 2x2 ops share FP32 vector units and FP32 matrix cores */
+#define SVX_MUX_VALU 1       /* every MUX2 op on the FP32 vector units */
+
+typedef struct svx_options {
+    int32_t size;            /* sizeof(svx_options) */
+    int32_t fuse;            /* SVX_FUSE_* */
+    int32_t dense_math;      /* SVX_DENSE_* */
+    int32_t kernel;          /* SVX_KERNEL_* */
+    int32_t diag_tables;     /* 1 (default): runs of diagonal ops as one table of phases; 0: never */
+    int32_t dense_only;      /* 1 (default): passes of 4-qubit units only on the 512-thread kernel; 0: never */
+    int32_t tile_scale;      /* split-FP16 only: 1 (default) per-tile pre-scaling, 0 per-column scaling */
+    int32_t tile_order;      /* SVX_ORDER_* */
+    int32_t mux_engine;      /* SVX_MUX_* */
+    int32_t fuse_width;      /* largest fused block in tile bits (1..5); 0: default 4 */
+    float fuse_cost4;        /* cost of one dense 4-qubit unit in FP32 FMAs per amplitude (a general MUX2
+                                op costs 8); 0: default; negative: zero cost (every block of >= 2 ops fuses) */
+    float fuse_cost5;        /* same for 5-qubit units */
+    int32_t reserved[4];
+} svx_options;
+
 typedef struct svx_caps {
     int32_t device;          /* HIP device ordinal */
     int32_t cu_count;        /* compute units */
@@ -64,10 +101,32 @@ int svx_run_segment(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t
  * fetch it), then runs it like svx_run_segment with one slice, complex64. */
 int svx_engine(void *st, int64_t ins_off, void *stream);
 
+/* svx_run_segment with explicit options (NULL = defaults). */
+int svx_run_segment_ex(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t dtype,
+                       const int32_t *qsv2, int64_t n_words, const svx_options *opts, void *stream);
+
+/* Fill `out` with the default options. */
+int svx_default_options(svx_options *out);
+
 /* Validate and upload a segment once; returns a handle for svx_run_prepared (no reference
- * counterpart: the reference re-uploads instructions per launch, host/sv/runtime.py:218-225). */
+ * counterpart: the reference re-uploads instructions per launch, host/sv/runtime.py:218-225).
+ * Equal streams with equal options share one handle (reference counted).  The caller owns one
+ * reference per successful call and gives it back with svx_release_prepared. */
 int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle);
-/* Launch a prepared segment: kernels only, no allocation or copy (safe under stream capture). */
+int svx_prepare_ex(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits,
+                   const svx_options *opts, int64_t *handle);
+/* Give back one reference.  A handle without references stays cached (so replays of the same
+ * stream need no upload) until the cache of unreferenced descriptors exceeds its byte limit; its
+ * device memory is freed once the last launch that used it has completed.  Using a handle after
+ * its last release is an error (SVX_E_ARG), never a fault. */
+int svx_release_prepared(int64_t handle);
+/* Byte limit of the cache of unreferenced descriptors (default 256 MiB); returns the bytes held by
+ * all prepared descriptors (referenced or not) in *held when non-NULL. */
+int svx_cache_limit(int64_t limit_bytes, int64_t *held);
+/* dtype and state bits a handle was prepared for (callers check their state against them). */
+int svx_prepared_shape(int64_t handle, int32_t *dtype, int32_t *state_bits);
+/* Launch a prepared segment: kernels only, no allocation or copy (safe under stream capture).
+ * `state` must hold 2^state_bits amplitudes of the handle's dtype (svx_prepared_shape). */
 int svx_run_prepared(void *state, int64_t handle, void *stream);
 
 /* Pick each pass's workgroup-to-tile order by measurement: every pass of the prepared segment that
@@ -76,8 +135,10 @@ int svx_run_prepared(void *state, int64_t handle, void *stream);
  * `state`, timed with HIP events (synchronously), and keeps the identity only when it is >= 3%
  * faster.  The order changes which workgroup takes which tile, never the arithmetic, so results
  * stay bit-identical.  `state` must be a state the segment may run on (its contents change: each
- * tunable pass is applied 2 (1 + reps) times); not concurrently with runs of the same handle.  No
- * reference counterpart (the FPGA engine's tile order is fixed by its reader). */
+ * tunable pass is applied 2 (1 + reps) times).  The chosen orders belong to the handle, which is
+ * shared by every caller that prepared the same stream with the same options (svx_run_segment
+ * included); launches running concurrently with the tuning use either order, with the same
+ * results.  No reference counterpart (the FPGA engine's tile order is fixed by its reader). */
 int svx_tune_prepared(void *state, int64_t handle, int32_t reps, void *stream);
 /* Continuous batching: run pass passes[i] of prepared segment handles[i] on states[i] (complex64,
  * i < n_items <= 4096) with one launch per kernel shape, the items' tiles side by side in one grid.

@@ -3,14 +3,11 @@
 The library turns a run of diagonal MUX2 records (diagonal M0 and M1: phases, controlled phases,
 controlled diagonals) into one table of 2^k phases indexed by the QSV2 tile index, hoisting the
 run's ops over the ops between them that share no bit with them.  The complex64 result is held to
-the reference's judge bar for complex64 states (bench/fpga/sv_maxwidth_ref.py:15-25: fidelity >=
-1 - 1e-6, max|da| <= 2e-5) against the complex128 golden model of the same records
-(host/sv/model.py:43-71); with tables switched off (SVX_DIAG=off) the same records must give the
+the FP32 rounding bound per amplitude of tests/_c64.py against the complex128 golden model of the
+same records (host/sv/model.py:43-71); with tables switched off the same records must give the
 same state to that bar, and the stats must show that tables were made.  A wrong hoist (a diagonal
 op moved over an op on one of its bits) changes amplitudes at O(1) and fails the bar.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -18,6 +15,7 @@ from oracle import judge, model
 from oracle.isa import MAGIC
 
 from test_gpu_pass import _mat, _op_words, _rand_state
+from _c64 import check_c64, prog_nops, seg_nops
 
 pytestmark = pytest.mark.gpu
 
@@ -49,25 +47,18 @@ def _segment(rng, L, k, n_passes, nops, p_other, max_m, ctrl_p=0.35):
 def _run(s0, seg, n, diag):
     import torch
     from u55c_statevector_30q_amd import libsvx
-    old = os.environ.get("SVX_DIAG")
-    os.environ["SVX_DIAG"] = diag
-    try:
+    with libsvx.options(diag_tables=(diag != "off")):
         h = libsvx.prepare(seg, libsvx.C64, n)
-        tables = sum(libsvx.prepared_pass_stats(h, p)["diag_tables"] for p in range(libsvx.prepared_info(h)[1]))
-        t = torch.from_numpy(s0.copy()).to("cuda")
-        libsvx.run_prepared(t, h)
-        torch.cuda.synchronize()
-        return t.cpu().numpy(), tables
-    finally:
-        if old is None:
-            os.environ.pop("SVX_DIAG", None)
-        else:
-            os.environ["SVX_DIAG"] = old
+    tables = sum(libsvx.prepared_pass_stats(h, p)["diag_tables"] for p in range(libsvx.prepared_info(h)[1]))
+    t = torch.from_numpy(s0.copy()).to("cuda")
+    libsvx.run_prepared(t, h)
+    torch.cuda.synchronize()
+    libsvx.release_prepared(h)
+    return t.cpu().numpy(), tables
 
 
-def _check(ref, got, what):
-    m = judge.metrics(ref, got)
-    assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 2e-5, (what, m)
+def _check(ref, got, what, nops):
+    check_c64(ref, got, what, nops)
 
 
 @pytest.mark.parametrize("n,k,nops,p_other", [(12, 10, 24, 0.0), (14, 12, 40, 0.2), (16, 13, 64, 0.1),
@@ -80,8 +71,8 @@ def test_diagonal_runs_match_golden_model(gpu, n, k, nops, p_other):
     got, tables = _run(s0.astype(np.complex64), seg, n, "auto")
     off, none = _run(s0.astype(np.complex64), seg, n, "off")
     assert tables >= 1 and none == 0, (tables, none)
-    _check(ref, got, ("tables", n, k, nops, p_other))
-    _check(ref, off, ("no tables", n, k, nops, p_other))
+    _check(ref, got, ("tables", n, k, nops, p_other), seg_nops(seg))
+    _check(ref, off, ("no tables", n, k, nops, p_other), seg_nops(seg))
 
 
 @pytest.mark.parametrize("n,k", [(14, 12), (16, 13)])
@@ -105,7 +96,7 @@ def test_diagonal_run_blocked_by_shared_bit(gpu, n, k):
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
     got, tables = _run(s0.astype(np.complex64), seg, n, "auto")
     assert tables >= 1
-    _check(ref, got, ("blocked", n, k))
+    _check(ref, got, ("blocked", n, k), seg_nops(seg))
 
 
 def test_cp_ladder_tables_and_fewer_mux_ops(gpu):
@@ -133,10 +124,11 @@ def test_cp_ladder_tables_and_fewer_mux_ops(gpu):
             tables += st["diag_tables"]
             mux += st["mux_ops"]
         libsvx.run_prepared(s, h)
+        libsvx.release_prepared(h)
     torch.cuda.synchronize()
     got = model.unpermute(s.cpu().numpy(), n, fmap)
     assert tables >= 5 and mux < sum(g.name == "cp" for g in qc.data) // 2, (tables, mux)
-    _check(gold, got, "cp ladder 20")
+    _check(gold, got, "cp ladder 20", prog_nops(prog))
 
 
 def test_diagonal_tables_in_batched_launches(gpu):
@@ -166,4 +158,4 @@ def test_diagonal_tables_in_batched_launches(gpu):
     for (h, s0, seg), st, a in zip(items, states, alone):
         got = st.cpu().numpy()
         assert np.array_equal(got, a)
-        _check(model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128), got, "batched tables")
+        _check(model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128), got, "batched tables", seg_nops(seg))

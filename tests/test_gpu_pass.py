@@ -1,16 +1,17 @@
 """Parity of the HIP pass kernel (libsvx svx_run_segment) with the CPU golden model.
 
 Tolerances: the complex128 kernel must match the complex128 oracle to 1e-12 (FMA contraction is
-the only difference); the complex64 kernel is judged like the reference judges c64 states
-(bench/fpga/sv_maxwidth_ref.py:15-25): fidelity >= 1 - 1e-6 and max|da| <= 2e-5 against the
-complex128 oracle run of the same records.  Ops that only move data (exact X, controlled exact X)
-must be bit-exact.
+the only difference); the complex64 kernel must stay within the FP32 rounding bound per amplitude
+of tests/_c64.py (C * nops * 2^-24 * (|a_i| + rms)) and 1 - F < 1e-9 against the complex128
+oracle run of the same records.  Ops that only move data (exact X, controlled exact X) must be
+bit-exact.
 """
 import numpy as np
 import pytest
 
 from oracle import judge, model
 from oracle.isa import MAGIC, OP_MUX2
+from _c64 import check_c64, seg_nops
 
 pytestmark = pytest.mark.gpu
 
@@ -92,9 +93,12 @@ def run_hip(state_np, words, P=1):
     return t.cpu().numpy()
 
 
-def _check_c64(ref128, got64, what):
-    m = judge.metrics(ref128, got64)
-    assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 2e-5, (what, m)
+def _check_c64(ref128, got64, what, seg):
+    check_c64(ref128, got64, what, seg_nops(seg))
+
+
+def _opts(**kw):
+    return _lib().options(**kw)
 
 
 CASES = [(4, 3, 1), (6, 4, 1), (8, 5, 2), (9, 6, 1), (10, 6, 1), (10, 10, 1), (11, 8, 2), (12, 12, 1),
@@ -114,7 +118,7 @@ def test_random_segments_c128_exact_and_c64(gpu, n, k, P):
         got128 = run_hip(s0, seg, P)
         assert np.max(np.abs(got128 - ref)) < 1e-12, (n, k, P)
     got64 = run_hip(s0.astype(np.complex64), seg, P)
-    _check_c64(ref, got64, (n, k, P))
+    _check_c64(ref, got64, (n, k, P), seg)
 
 
 @pytest.mark.parametrize("k", [1, 2, 3, 5, 8])
@@ -199,7 +203,7 @@ def test_engine_reference_abi(gpu):
     torch.cuda.synchronize()
     got = d.cpu().numpy()[:(1 << L) * 8].view(np.complex64)
     ref = model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128)
-    _check_c64(ref, got, "engine")
+    _check_c64(ref, got, "engine", seg)
 
 
 def test_prepared_replay_is_identical(gpu):
@@ -221,55 +225,62 @@ def test_prepared_replay_is_identical(gpu):
 
 
 @pytest.mark.parametrize("n,k,max_m", [(21, 12, 4), (21, 12, 7), (22, 13, 7), (22, 13, 8)])
-def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m, monkeypatch):
-    """The double-buffered kernel (one QSV2 tile per LDS buffer, loader wave + worker waves), the
-    persistent MUX2 kernel and the single-buffer kernel apply the same ops in the same order with the
-    same arithmetic: their results are bit-identical, and all match the complex128 golden model."""
+def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m):
+    """The double-buffered kernel (one QSV2 tile per LDS buffer, loader wave + worker waves) and
+    the single-buffer kernel apply the same ops in the same order with the same arithmetic (every
+    op on the vector units): their results are bit-identical, and both match the complex128
+    golden model."""
     rng = np.random.default_rng(n * 100 + k + max_m)
     seg = random_segment(rng, n, k, n_passes=2, max_ops=10, max_m=max_m)
     s0 = _rand_state(rng, n, np.complex64)
     ref = model.run_segment(s0.astype(np.complex128), 1, seg, np.complex128)
     got = {}
-    monkeypatch.setenv("SVX_FUSE", "off")
-    monkeypatch.setenv("SVX_DIAG", "off")   # diagonal tables: single-buffer kernel only
-    for pick in ("dma", "pers", "v1"):
-        monkeypatch.setenv("SVX_PASS_KERNEL", pick)
-        got[pick] = run_hip(s0, seg)
-        _check_c64(ref, got[pick], (n, k, max_m, pick))
-    assert np.array_equal(got["dma"], got["v1"])
-    assert np.array_equal(got["pers"], got["v1"])
+    for pick in ("dma", "single"):
+        with _opts(fuse="off", diag_tables=False, mux_engine="valu", kernel=pick):
+            got[pick] = run_hip(s0, seg)
+        _check_c64(ref, got[pick], (n, k, max_m, pick), seg)
+    assert np.array_equal(got["dma"], got["single"])
 
 
-@pytest.mark.parametrize("fuse,cost", [("off", None), ("auto", None), ("on", None), ("on", "0,0,5"),
-                                       ("on", "0,0,3"), ("on", "0,0,1"), ("dense4", None)])
+FUSE_SETTINGS = {"off": dict(fuse="off"), "auto": dict(fuse="auto"),
+                 "all5": dict(fuse="auto", fuse_cost4=-1, fuse_cost5=-1, fuse_width=5),
+                 "all3": dict(fuse="auto", fuse_cost4=-1, fuse_cost5=-1, fuse_width=3),
+                 "all1": dict(fuse="auto", fuse_cost4=-1, fuse_cost5=-1, fuse_width=1),
+                 "dense4": dict(fuse="dense4"), "dense4_split16": dict(fuse="dense4", dense_math="split16")}
+
+
+@pytest.mark.parametrize("fuse", list(FUSE_SETTINGS))
 @pytest.mark.parametrize("n,k,max_m", [(12, 10, 4), (16, 12, 6), (20, 13, 7), (21, 12, 4), (22, 13, 7)])
-def test_pass_fusion_matches_oracle(gpu, n, k, max_m, fuse, cost, monkeypatch):
+def test_pass_fusion_matches_oracle(gpu, n, k, max_m, fuse):
     """Pass fusion (MUX2 ops of a pass clustered into dense units on the matrix cores): every
     setting, from off to fusing every block of 1, 3 or 5 tile bits, matches the complex128 golden
-    model; ops with controls and several kinds of matrices, including controls wider than a block."""
-    monkeypatch.setenv("SVX_FUSE", fuse)
-    if cost:
-        monkeypatch.setenv("SVX_FUSE_COST", cost)
+    model; ops with controls and several kinds of matrices, including controls wider than a block.
+    FP32 units are held to the FP32 bound; the split-FP16 opt-in only to FP32-level fidelity."""
     rng = np.random.default_rng(n * 31 + k + max_m)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=40, max_m=max_m, ctrl_p=0.12)
     s0 = _rand_state(rng, n)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
-    got = run_hip(s0.astype(np.complex64), seg)
-    _check_c64(ref, got, (n, k, max_m, fuse, cost))
+    with _opts(**FUSE_SETTINGS[fuse]):
+        got = run_hip(s0.astype(np.complex64), seg)
+    if fuse.endswith("split16"):
+        m = judge.metrics(ref, got)
+        assert m["one_minus_f"] < 1e-9, (n, k, fuse, m)
+    else:
+        _check_c64(ref, got, (n, k, max_m, fuse), seg)
 
 
 @pytest.mark.parametrize("n,k", [(16, 12), (22, 13)])
-def test_pass_fusion_of_permutations_is_exact(gpu, n, k, monkeypatch):
-    """Blocks of exact X / controlled-X ops fuse into 0/1 matrices, which run on the exact FP32
-    matrix-core path (single-buffer and double-buffered kernels): the result equals the c64 golden
-    model bit for bit."""
-    monkeypatch.setenv("SVX_FUSE", "on")
-    monkeypatch.setenv("SVX_FUSE_COST", "0,0,5")
+def test_pass_fusion_of_permutations_is_exact(gpu, n, k):
+    """Blocks of exact X / controlled-X ops fuse into 0/1 matrices, which run on the FP32 matrix
+    cores exactly (also when split FP16 is requested): the result equals the c64 golden model bit
+    for bit."""
     rng = np.random.default_rng(5)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=40, kinds=[("x", "x"), ("i", "x")], max_m=6)
     s0 = _rand_state(rng, n, np.complex64)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex64)
-    assert np.array_equal(run_hip(s0, seg), ref)
+    for math in ("f32", "split16"):
+        with _opts(**FUSE_SETTINGS["all5"], dense_math=math):
+            assert np.array_equal(run_hip(s0, seg), ref), math
 
 
 def _gain_segment(rng, L, k, n_passes, n_ops, gain, max_m=6):
@@ -291,64 +302,71 @@ def _gain_segment(rng, L, k, n_passes, n_ops, gain, max_m=6):
 
 
 @pytest.mark.parametrize("gain,scale", [(1.0, 1.0), (1.0, 1e6), (1.0, 1e-25), (1.15, 1.0), (2.0, 1.0), (0.5, 1e3)])
-@pytest.mark.parametrize("tile_scale", ["auto", "off"])
-def test_pass_fusion_tile_scaling(gpu, gain, scale, tile_scale, monkeypatch):
-    """Split-FP16 dense groups on a pre-scaled tile (the pass kernel scales each tile by a power of
-    two from the tile maximum and the host's growth bound of the pass's matrices) and with
-    per-column scaling: unnormalised states far from unit norm and non-unitary ops that grow or
-    shrink the amplitudes within a pass match the complex128 golden model (judged on the
-    normalised states)."""
-    monkeypatch.setenv("SVX_FUSE", "on")
-    monkeypatch.setenv("SVX_TILE_SCALE", tile_scale)
+@pytest.mark.parametrize("math", ["f32", "split16_tile", "split16_col"])
+def test_pass_fusion_tile_scaling(gpu, gain, scale, math):
+    """Dense groups on unnormalised states far from unit norm and non-unitary ops that grow or
+    shrink the amplitudes within a pass: FP32 matrix cores, and the split-FP16 opt-in on a
+    pre-scaled tile (the pass kernel scales each tile by a power of two from the tile maximum and
+    the host's growth bound of the pass's matrices) and with per-column scaling, all match the
+    complex128 golden model."""
     rng = np.random.default_rng(int(gain * 100) + int(np.log10(scale)) + 50)
     n, k = 18, 13
     seg = _gain_segment(rng, n, k, n_passes=3, n_ops=24, gain=gain)
     s0 = _rand_state(rng, n) * scale
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
-    got = run_hip(s0.astype(np.complex64), seg)
+    kw = dict(fuse="auto", fuse_cost4=-1, fuse_cost5=-1, fuse_width=5)
+    if math != "f32":
+        kw.update(dense_math="split16", tile_scale=(math == "split16_tile"))
+    with _opts(**kw):
+        got = run_hip(s0.astype(np.complex64), seg)
     assert np.all(np.isfinite(got))
-    _check_c64(ref, got, (gain, scale, tile_scale))
+    if math == "f32":
+        _check_c64(ref, got, (gain, scale, math), seg)
+    else:
+        m = judge.metrics(ref, got)
+        assert m["one_minus_f"] < 1e-9, (gain, scale, math, m)
     nr = np.linalg.norm(ref)
     assert abs(np.linalg.norm(got.astype(np.complex128)) / nr - 1) < 1e-5
 
 
-@pytest.mark.parametrize("pick", ["dma", "dma5"])
 @pytest.mark.parametrize("n,k", [(21, 12), (22, 13)])
-def test_pass_fusion_double_buffered_kernels(gpu, n, k, pick, monkeypatch):
-    """Pass fusion on the double-buffered kernel's dense instances (8 worker waves with 4-bit
-    register groups, or 4 worker waves with 5-bit groups) against the complex128 golden model."""
-    if pick == "dma5" and k != 13:
-        pytest.skip("the 5-bit-group double-buffered kernel exists for 2^13-amplitude tiles")
-    monkeypatch.setenv("SVX_PASS_KERNEL", pick)
-    monkeypatch.setenv("SVX_FUSE", "on")
+def test_pass_fusion_double_buffered_kernel(gpu, n, k):
+    """Pass fusion on the double-buffered kernel's dense instance (worker waves with 4-bit register
+    groups, XOR-swizzled LDS tile) against the complex128 golden model."""
     rng = np.random.default_rng(7 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.12)
     s0 = _rand_state(rng, n)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
-    got = run_hip(s0.astype(np.complex64), seg)
-    _check_c64(ref, got, (n, k, pick))
+    with _opts(kernel="dma", fuse="auto", fuse_cost4=-1, fuse_cost5=-1):
+        got = run_hip(s0.astype(np.complex64), seg)
+    _check_c64(ref, got, (n, k, "dma"), seg)
 
 
+@pytest.mark.parametrize("math", ["f32", "split16"])
 @pytest.mark.parametrize("n,k", [(18, 12), (20, 13), (22, 13)])
-def test_dense_only_kernel_matches_general_kernel(gpu, n, k, monkeypatch):
-    """Every op of every pass in a dense 4-qubit unit (SVX_FUSE=dense4) runs on the 512-thread
-    dense-only kernel; it is bit-identical to the general dense kernel (SVX_D4=off: same units, same
-    arithmetic, other thread count; unit pairs off) and matches the complex128 golden model."""
+def test_dense_only_kernel_matches_general_kernel(gpu, n, k, math):
+    """Every op of every pass in a dense 4-qubit unit (fuse=dense4) runs on the 512-thread
+    dense-only kernel; it is bit-identical to the general dense kernel (dense_only off: same units,
+    same arithmetic, other thread count) and matches the complex128 golden model."""
     from u55c_statevector_30q_amd import libsvx
-    monkeypatch.setenv("SVX_FUSE", "dense4")
-    monkeypatch.setenv("SVX_PAIR", "off")
     rng = np.random.default_rng(3 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)
     s0 = _rand_state(rng, n)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
-    got = {}
-    for d4 in ("auto", "off"):
-        monkeypatch.setenv("SVX_D4", d4)
-        got[d4] = run_hip(s0.astype(np.complex64), seg)
-        _check_c64(ref, got[d4], (n, k, d4))
-    h = libsvx.prepare(seg, libsvx.C64, n)
-    names = [libsvx.prepared_kernel(h, p) for p in range(3)]
-    assert np.array_equal(got["auto"], got["off"]), names
+    got, names = {}, {}
+    for d4 in (True, False):
+        with _opts(fuse="dense4", dense_math=math, dense_only=d4):
+            got[d4] = run_hip(s0.astype(np.complex64), seg)
+            h = libsvx.prepare(seg, libsvx.C64, n)
+            names[d4] = [libsvx.prepared_kernel(h, p) for p in range(3)]
+            libsvx.release_prepared(h)
+        if math == "f32":
+            _check_c64(ref, got[d4], (n, k, d4), seg)
+        else:
+            assert judge.metrics(ref, got[d4])["one_minus_f"] < 1e-9
+    assert any(nm.startswith("k_pass_d4<") for nm in names[True]), names
+    assert not any(nm.startswith("k_pass_d4<") for nm in names[False]), names
+    assert np.array_equal(got[True], got[False]), names
 
 
 def _dense_pass_words(rng, n, k, hi, unit_qubits):
@@ -367,87 +385,54 @@ def _dense_pass_words(rng, n, k, hi, unit_qubits):
     return (np.asarray(w, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
 
 
-def _pair_runs(libsvx, seg, n, s0, monkeypatch):
-    got, pairs = {}, {}
-    for pm in ("on", "off"):
-        monkeypatch.setenv("SVX_PAIR", pm)
-        h = libsvx.prepare(seg, libsvx.C64, n)
-        pairs[pm] = sum(libsvx.prepared_pass_stats(h, p)["dense_pairs"] for p in range(libsvx.prepared_info(h)[1]))
-        got[pm] = run_hip(s0.astype(np.complex64), seg)
-    return got, pairs
-
-
-@pytest.mark.parametrize("n,k,max_m", [(20, 13, 6), (22, 13, 7), (21, 12, 5)])
-def test_dense_unit_pairs_match_oracle(gpu, n, k, max_m, monkeypatch):
-    """SVX_PAIR=on: consecutive pre-scaled 4-qubit units on disjoint bits share one LDS round trip on
-    the dense-only kernel (unit A transposed on the matrix cores, its accumulators turned into unit B's
-    operand by 16- and 32-lane register swaps): pairs are formed, the result matches the complex128
-    golden model and the unpaired form to the complex64 bar."""
-    from u55c_statevector_30q_amd import libsvx
-    monkeypatch.setenv("SVX_FUSE", "dense4")
-    rng = np.random.default_rng(11 * n + k + max_m)
-    seg = random_segment(rng, n, k, n_passes=4, max_ops=36, max_m=max_m, ctrl_p=0.05)
-    s0 = _rand_state(rng, n)
-    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
-    got, pairs = _pair_runs(libsvx, seg, n, s0, monkeypatch)
-    assert pairs["on"] > 0 and pairs["off"] == 0, pairs
-    for pm in got:
-        _check_c64(ref, got[pm], (n, k, max_m, pm))
-    m = judge.metrics(got["off"].astype(np.complex128), got["on"])
-    assert m["one_minus_f"] < 1e-9, m
-
-
 @pytest.mark.parametrize("units", [
     [(0, 1, 2, 3), (4, 5, 6, 7)],                          # tile bit 0 in the first unit: 16-byte loads
-    [(5, 6, 7, 8), (9, 10, 11, 12)],                       # tile bit 0 a context bit
-    [(3, 0, 9, 6), (12, 2, 7, 10), (1, 4, 5, 8), (11, 0, 2, 3)],   # two pairs, bit 0 inside, unordered bits
-    [(9, 10, 11, 12), (1, 2, 3, 4), (0, 5, 6, 7)],         # a pair, then a single unit
+    [(5, 6, 7, 8), (9, 10, 11, 12)],                       # tile bit 0 outside every unit
+    [(3, 0, 9, 6), (12, 2, 7, 10), (1, 4, 5, 8), (11, 0, 2, 3)],   # bit 0 inside, unordered bits
 ])
-def test_dense_unit_pairs_on_chosen_bits(gpu, units, monkeypatch):
-    """Fused-form dense records on chosen tile bits (20 qubits, 2^13 tiles, 4 high qubits): both load
-    paths of the paired form, pairs followed by a single unit, bits in any order; vs the golden model."""
-    from u55c_statevector_30q_amd import libsvx
+def test_dense_units_on_chosen_bits(gpu, units):
+    """Fused-form dense records on chosen tile bits (20 qubits, 2^13 tiles, 4 high qubits), both
+    16-byte LDS paths of the dense-only kernel, bits in any order; vs the golden model, FP32 units
+    held to the FP32 bound."""
     n, k = 20, 13
     rng = np.random.default_rng(sum(sum(u) for u in units))
     seg = _dense_pass_words(rng, n, k, [16, 17, 18, 19], units)
     s0 = _rand_state(rng, n)
     ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
-    got, pairs = _pair_runs(libsvx, seg, n, s0, monkeypatch)
-    assert pairs["on"] >= 1 and pairs["off"] == 0, pairs
-    for pm in got:
-        m = judge.metrics(ref, got[pm])
-        assert m["one_minus_f"] < 1e-9 and m["max_abs_err"] < 2e-6, (pm, m)
+    got = run_hip(s0.astype(np.complex64), seg)
+    _check_c64(ref, got, units, seg)
 
 
-@pytest.mark.parametrize("tile_scale", ["on", "off"])
+@pytest.mark.parametrize("math", ["f32", "split16_tile", "split16_col"])
 @pytest.mark.parametrize("n,k", [(24, 13), (23, 12)])
-def test_dense_only_kernel_large_states(gpu, n, k, tile_scale, monkeypatch):
-    """The dense-only kernel on states of 2^11 tiles, with pre-scaled tiles and with per-column
-    scaling: bit-identical to the general dense kernel (SVX_D4=off; unit pairs off), norm kept."""
+def test_dense_only_kernel_large_states(gpu, n, k, math):
+    """The dense-only kernel on states of 2^11 tiles (FP32, and split FP16 with pre-scaled tiles and
+    with per-column scaling): bit-identical to the general dense kernel, norm kept."""
     from u55c_statevector_30q_amd import libsvx
-    monkeypatch.setenv("SVX_FUSE", "dense4")
-    monkeypatch.setenv("SVX_PAIR", "off")
-    monkeypatch.setenv("SVX_TILE_SCALE", tile_scale)
     rng = np.random.default_rng(5 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)
     s0 = _rand_state(rng, n, np.complex64)
+    kw = dict(fuse="dense4")
+    if math != "f32":
+        kw.update(dense_math="split16", tile_scale=(math == "split16_tile"))
     got, names = {}, {}
-    for d4 in ("auto", "off"):
-        monkeypatch.setenv("SVX_D4", d4)
-        h = libsvx.prepare(seg, libsvx.C64, n)
-        names[d4] = [libsvx.prepared_kernel(h, p) for p in range(libsvx.prepared_info(h)[1])]
-        got[d4] = run_hip(s0, seg)
-    assert any(nm.startswith("k_pass_d4<") for nm in names["auto"]), names
-    assert not any(nm.startswith("k_pass_d4<") for nm in names["off"]), names
-    assert np.array_equal(got["auto"], got["off"])
-    assert abs(np.linalg.norm(got["auto"].astype(np.complex128)) - 1) < 1e-5
+    for d4 in (True, False):
+        with _opts(**kw, dense_only=d4):
+            h = libsvx.prepare(seg, libsvx.C64, n)
+            names[d4] = [libsvx.prepared_kernel(h, p) for p in range(libsvx.prepared_info(h)[1])]
+            libsvx.release_prepared(h)
+            got[d4] = run_hip(s0, seg)
+    assert any(nm.startswith("k_pass_d4<") for nm in names[True]), names
+    assert not any(nm.startswith("k_pass_d4<") for nm in names[False]), names
+    assert np.array_equal(got[True], got[False])
+    assert abs(np.linalg.norm(got[True].astype(np.complex128)) - 1) < 1e-5
 
 
 @pytest.mark.parametrize("dt", ["c64", "c128"])
-def test_tile_order_tuning_keeps_results(gpu, dt, monkeypatch):
+def test_tile_order_tuning_keeps_results(gpu, dt):
     """svx_tune_prepared picks each pass's workgroup-to-tile order by timing; the order only changes
-    which workgroup takes which tile, so the tuned run, the default order and the identity order
-    (SVX_TILE_ORDER=0) give bit-identical states, and the stats report the order in use."""
+    which workgroup takes which tile, so the tuned run, the default order and the fixed identity
+    order give bit-identical states, and the stats report the order in use."""
     from u55c_statevector_30q_amd import libsvx
     torch = _torch()
     n, k = 22, 13
@@ -471,13 +456,13 @@ def test_tile_order_tuning_keeps_results(gpu, dt, monkeypatch):
     orders = [libsvx.prepared_pass_stats(h, p)["tile_order"] for p in range(libsvx.prepared_info(h)[1])]
     assert set(orders) <= {-2, 0}, orders
     tuned = run(h)
-    monkeypatch.setenv("SVX_TILE_ORDER", "0")
-    hi = libsvx.prepare(seg, code, n)
+    with _opts(tile_order="identity"):
+        hi = libsvx.prepare(seg, code, n)
     assert all(libsvx.prepared_pass_stats(hi, p)["tile_order"] == 0 for p in range(libsvx.prepared_info(hi)[1]))
     ident = run(hi)
     assert np.array_equal(before, tuned) and np.array_equal(before, ident)
     if dt == "c64":
-        _check_c64(ref, before, (n, k))
+        _check_c64(ref, before, (n, k), seg)
     else:
         assert np.max(np.abs(before - ref)) < 1e-12
     with pytest.raises(ValueError):

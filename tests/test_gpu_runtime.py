@@ -5,6 +5,8 @@ circuits on the GPU against the gate-level complex128 simulator (n <= 12), the g
 import numpy as np
 import pytest
 
+from _c64 import check_c64, prog_nops
+
 from oracle import dense, judge, model
 from u55c_statevector_30q_amd import circuit as C
 from u55c_statevector_30q_amd import compiler as K
@@ -68,8 +70,7 @@ def test_cpu20_workload_vs_golden_model(gpu):
     ref = model.run_program(prog, 20, 1, fmap, ctype=np.complex128)
     for target in ("mi355x", "reference"):
         out = SVRuntime(target=target).run(qc, shots=0)
-        m = judge.metrics(ref, out["state"])
-        assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 1e-5, (target, m)
+        check_c64(ref, out["state"], target, st.n_ops)
 
 
 def test_gpu30_workload_full_size(gpu):
@@ -110,7 +111,7 @@ def test_cp_ladder_product_state_and_bit_reversal(gpu, n):
         s0 = np.kron(amps[q], s0)
     gold = model.run_program(prog, n, 1, fmap, state=s0.copy(), ctype=np.complex128)
     rev = list(range(n))[::-1]
-    for dt, tol in ((torch.complex128, 1e-12), (torch.complex64, 2e-6)):
+    for dt, tol in ((torch.complex128, 1e-12), (torch.complex64, None)):
         s = torch.empty(1 << n, dtype=dt, device="cuda")
         libsvx.product_state(s, amps)
         for _, w in prog:
@@ -120,7 +121,10 @@ def test_cp_ladder_product_state_and_bit_reversal(gpu, n):
         torch.cuda.synchronize()
         got, before = t.cpu().numpy(), s.cpu().numpy()
         assert np.array_equal(got, model.remap(before, n, rev))
-        assert np.max(np.abs(before - gold)) < tol, dt
+        if tol is None:
+            check_c64(gold, before, n, prog_nops(prog))
+        else:
+            assert np.max(np.abs(before - gold)) < tol, dt
 
 
 def test_autotuner_and_every_form(gpu, tmp_path):
@@ -147,5 +151,4 @@ def test_autotuner_and_every_form(gpu, tmp_path):
             tuner.records[key] = dict(k=k, form=form)
             out = rt.run(qc, shots=0)
             assert rt.form == form and out["k"] == k
-            m = judge.metrics(gold, out["state"])
-            assert m["one_minus_f"] < 1e-6 and m["max_abs_err"] < 1e-5, (k, form, m)
+            check_c64(gold, out["state"], (k, form), st.n_ops)

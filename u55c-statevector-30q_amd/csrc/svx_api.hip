@@ -13,6 +13,7 @@
 #include "svx_half.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,6 +52,113 @@ int hipfail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hipfail(e_, #x); \
     } while (0)
 
+// ------------------------------------------------------------------------------------------ options
+// Normalised preparation options (the cache key of a prepared segment): every field explicit.
+struct Opts {
+    int32_t fuse, dense_math, kernel, diag, dense_only, tile_scale, tile_order, mux, width;
+    float cost4, cost5;
+};
+
+void default_opts(svx_options *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->size = (int32_t)sizeof(svx_options);
+    o->fuse = SVX_FUSE_AUTO;
+    o->dense_math = SVX_DENSE_F32;
+    o->kernel = SVX_KERNEL_AUTO;
+    o->diag_tables = 1;
+    o->dense_only = 1;
+    o->tile_scale = 1;
+    o->tile_order = SVX_ORDER_TUNABLE;
+    o->mux_engine = SVX_MUX_AUTO;
+}
+
+int normalize_opts(const svx_options *in, Opts *o) {
+    svx_options d;
+    default_opts(&d);
+    if (in && in->size != (int32_t)sizeof(svx_options))
+        return fail(SVX_E_ARG, "svx_options.size must be sizeof(svx_options)");
+    const svx_options &s = in ? *in : d;
+    std::memset(o, 0, sizeof(*o));
+    if (s.fuse < SVX_FUSE_AUTO || s.fuse > SVX_FUSE_DENSE4) return fail(SVX_E_ARG, "bad svx_options.fuse");
+    if (s.dense_math != SVX_DENSE_F32 && s.dense_math != SVX_DENSE_SPLIT16)
+        return fail(SVX_E_ARG, "bad svx_options.dense_math");
+    if (s.kernel < SVX_KERNEL_AUTO || s.kernel > SVX_KERNEL_DMA) return fail(SVX_E_ARG, "bad svx_options.kernel");
+    if (s.tile_order < SVX_ORDER_TUNABLE || s.tile_order > SVX_ORDER_XCD)
+        return fail(SVX_E_ARG, "bad svx_options.tile_order");
+    if (s.mux_engine != SVX_MUX_AUTO && s.mux_engine != SVX_MUX_VALU) return fail(SVX_E_ARG, "bad svx_options.mux_engine");
+    if (s.fuse_width < 0 || s.fuse_width > 5) return fail(SVX_E_ARG, "svx_options.fuse_width must be 0..5");
+    o->fuse = s.fuse;
+    o->dense_math = s.dense_math;
+    o->kernel = s.kernel;
+    o->diag = s.diag_tables ? 1 : 0;
+    o->dense_only = s.dense_only ? 1 : 0;
+    o->tile_scale = s.tile_scale ? 1 : 0;
+    o->tile_order = s.tile_order;
+    o->mux = s.mux_engine;
+    o->width = s.fuse_width ? s.fuse_width : 4;
+    // dense-unit costs in FP32 FMAs per amplitude (a general MUX2 op costs 8 on the vector units):
+    // split-FP16 units run at the vector units' rate of about one general op each; an FP32 4-qubit
+    // unit (64 matrix-core FMAs per amplitude) measured like about four
+    const bool f32 = s.dense_math == SVX_DENSE_F32;
+    o->cost4 = s.fuse_cost4 > 0 ? s.fuse_cost4 : (f32 ? 32.f : 8.f);
+    o->cost5 = s.fuse_cost5 > 0 ? s.fuse_cost5 : (f32 ? 64.f : 16.f);
+    if (s.fuse_cost4 < 0 || s.fuse_cost5 < 0) {   // explicit zero-cost request: fuse every block
+        o->cost4 = std::max(0.f, s.fuse_cost4);
+        o->cost5 = std::max(0.f, s.fuse_cost5);
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------ descriptor arena
+// Prepared descriptors live in library-owned device slabs, carved into power-of-two blocks with one
+// free list per size; a released descriptor's block is reused, never hipFree'd (hipFree would wait
+// for the device).  Blocks above the slab size get their own allocation.
+class Arena {
+  public:
+    static constexpr size_t kSlab = size_t(16) << 20;
+    static constexpr int kMinLog = 12;   // 4 KiB
+    int alloc(size_t bytes, void **out, size_t *got) {
+        int lg = kMinLog;
+        while ((size_t(1) << lg) < bytes) ++lg;
+        const size_t sz = size_t(1) << lg;
+        if (sz > kSlab) {
+            HIPCHK(hipMalloc(out, sz));
+            big_.insert(*out);
+            held_ += sz;
+            *got = sz;
+            return 0;
+        }
+        auto &fl = free_[lg];
+        if (fl.empty()) {
+            void *slab = nullptr;
+            HIPCHK(hipMalloc(&slab, kSlab));
+            held_ += kSlab;
+            for (size_t off = 0; off < kSlab; off += sz) fl.push_back(static_cast<unsigned char *>(slab) + off);
+        }
+        *out = fl.back();
+        fl.pop_back();
+        *got = sz;
+        return 0;
+    }
+    void release(void *p, size_t sz) {
+        if (big_.count(p)) {
+            (void)hipFree(p);
+            big_.erase(p);
+            held_ -= sz;
+            return;
+        }
+        int lg = kMinLog;
+        while ((size_t(1) << lg) < sz) ++lg;
+        free_[lg].push_back(p);
+    }
+    size_t held() const { return held_; }
+
+  private:
+    std::map<int, std::vector<void *>> free_;
+    std::set<void *> big_;
+    size_t held_ = 0;
+};
+
 struct DeviceState {
     bool ready = false;
     svx_caps caps{};
@@ -69,6 +177,7 @@ struct DeviceState {
     bool batch_used[kBatchSlots] = {};
     int batch_next = 0;
     std::mutex mu;
+    Arena arena;                   // prepared descriptors (guarded by g_prep_mu)
 };
 std::mutex g_mu;
 std::map<int, std::unique_ptr<DeviceState>> g_dev;
@@ -230,37 +339,52 @@ bool same(const T *a, const T *b) {
 struct LaunchInfo {
     bool small;
     bool dma;             // double-buffered complex64 kernel (k_pass_dma)
-    bool pers;            // persistent single-buffer complex64 kernel (k_pass_pers)
-    int dma_variant;      // 0: default worker layout; 100: 5-bit register groups, 4 worker waves
     int K;
     uint64_t ntiles;
     uint32_t grid;
+    uint32_t block;       // threads per workgroup
     size_t lds;
     int64_t pass_index;   // index into the PassDev array
     bool empty;           // no op changes the state (skip launch)
-    bool dense;           // pass holds dense (fused-form) groups: MFMA instance of k_pass
+    bool dense;           // pass holds dense (fused-form) groups or tables: the dense instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
     bool tunable;         // single-buffer / dense-only launch whose tile order svx_tune_prepared may switch
-    int32_t stats[7];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32), pairs,
-                          // diagonal tables
+    int32_t stats[7];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32),
+                          // MUX2 ops on the matrix cores, diagonal tables
 };
 
 struct Prepared {
     int dtype = 0, nbits = 0;
     std::vector<int32_t> words;
-    std::string pick;           // SVX_PASS_KERNEL at preparation time
+    Opts opts{};
+    uint64_t key = 0;
     void *dev = nullptr;        // [PassDev x P][PassDev x P, identity tile order][GroupDev x G][OpDev<T> x O][operands]
+    size_t dev_bytes = 0;       // arena block size
     size_t pass_off = 0, alt_off = 0, group_off = 0, op_off = 0, w_off = 0;
     std::vector<LaunchInfo> launches;
-    std::vector<int8_t> alt;    // per pass: 1 = launch with the identity tile order (set by svx_tune_prepared)
+    // per pass: 1 = launch with the identity tile order (set by svx_tune_prepared, read by launches)
+    std::unique_ptr<std::atomic<uint8_t>[]> alt;
     int torder = -2;            // default workgroup -> tile order of the passes (PassDev::torder)
     int device = 0;
+    int refs = 0;               // references held by callers (svx_prepare / svx_release_prepared)
+    uint64_t lru = 0;           // last release order (unreferenced entries are evicted oldest first)
+    hipEvent_t last_use = nullptr;   // recorded after each launch sequence (not under stream capture)
 };
 
+// Registry of prepared segments.  handle = index + 1; an evicted handle's slot stays empty, so a
+// stale handle is an error, never another segment.
 std::mutex g_prep_mu;
-std::vector<std::unique_ptr<Prepared>> g_prepared;              // handle = index + 1
-std::multimap<uint64_t, int64_t> g_prep_index;                  // hash -> handle
-size_t g_prep_bytes = 0;
+std::vector<std::unique_ptr<Prepared>> g_prepared;
+std::multimap<uint64_t, int64_t> g_prep_index;   // content hash -> handle
+size_t g_unref_bytes = 0;                         // descriptor bytes of unreferenced entries
+size_t g_unref_limit = size_t(256) << 20;
+uint64_t g_lru_clock = 0;
+
+Prepared *lookup_locked(int64_t handle) {
+    if (handle < 1 || handle > (int64_t)g_prepared.size()) return nullptr;
+    Prepared *p = g_prepared[handle - 1].get();
+    return (p && p->refs > 0) ? p : nullptr;
+}
 
 uint64_t fnv1a(const int32_t *w, int64_t n, uint64_t seed) {
     uint64_t h = 1469598103934665603ull ^ seed;
@@ -631,63 +755,35 @@ void diag_runs(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
 }
 
 template <typename T>
-int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
+int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Prepared *pr,
+                   std::vector<unsigned char> *blob) {
     const int k = seg.k;
     const int maxK = dtype == SVX_C64 ? kMaxKc64 : kMaxKc128;
     if (k > maxK)
         return fail(SVX_E_CAPACITY, "tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS tile limit 2^" +
                                         std::to_string(maxK));
     const bool small = nbits <= kSmallMaxBits;
-    // kernel choice per pass.  "auto" runs every pass on the single-buffer kernel (k_pass: two
-    // workgroups per CU, non-temporal 16-byte HBM access, 5-bit register groups, dense groups on the
-    // matrix cores), which measured fastest for light and heavy passes alike.  The double-buffered
-    // DMA kernel (k_pass_dma: loader wave + worker waves; complex64, K = k in {12, 13}, enough tiles
-    // to keep every CU busy) and the persistent variant stay selectable: SVX_PASS_KERNEL=v1|dma|
-    // dma5|pers|auto.
-    const char *env = std::getenv("SVX_PASS_KERNEL");
-    const std::string pick = env ? env : "auto";
-    // dense records: split-FP16 matrix cores unless every entry is 0 / +-1 (then FP32 matrix cores,
-    // exact); SVX_DENSE_MATH=f32|f16 forces one of them
-    const char *denv = std::getenv("SVX_DENSE_MATH");
-    const std::string dense_math = denv ? denv : "auto";
-    // pass fusion of MUX2 ops into dense units (complex64): "auto"/"on" fuse every pass where a
-    // block's MUX2 cost exceeds a dense op's, "off" none; SVX_FUSE_COST="c4,c5[,width]" overrides
-    // the dense-op costs and the block width
-    const char *fenv = std::getenv("SVX_FUSE");
-    const std::string fuse_mode = fenv ? fenv : "auto";
-    // tile scaling for split-FP16 dense groups in the single-buffer kernel (SVX_TILE_SCALE=off: per-column scaling)
-    const char *tenv = std::getenv("SVX_TILE_SCALE");
-    const bool tile_scale_ok = !(tenv && std::string(tenv) == "off");
+    // kernel choice per pass (SVX_KERNEL_AUTO): dense-only passes of 4-qubit units -> k_pass_d4,
+    // everything else -> the single-buffer kernel k_pass (two workgroups per CU, non-temporal 16-byte
+    // HBM access, register groups, dense groups on the matrix cores).  SVX_KERNEL_DMA selects the
+    // double-buffered kernel (loader wave + worker waves; complex64, k in {12, 13}).
+    const bool pick_dma = opt.kernel == SVX_KERNEL_DMA;
+    const bool pick_single = opt.kernel == SVX_KERNEL_SINGLE;
+    const bool dense_f32 = opt.dense_math == SVX_DENSE_F32;
+    const bool tile_scale_ok = opt.tile_scale != 0;
     FuseCfg fcfg;
-    if (fuse_mode == "dense4") {   // every block of at most 4 tile bits a dense unit
+    fcfg.width = std::min(5, std::max(1, opt.width));
+    fcfg.cost4 = opt.cost4;
+    fcfg.cost5 = opt.cost5;
+    if (opt.fuse == SVX_FUSE_DENSE4) {   // every block of at most 4 tile bits a dense unit
         fcfg.width = 4;
         fcfg.all_dense = true;
     }
-    const char *d4env = std::getenv("SVX_D4");
-    const std::string d4_mode = d4env ? d4env : "auto";
-    // pairs of 4-qubit units on disjoint bits in one LDS round trip (dense-only passes, SVX_PAIR=on): the
-    // paired form measured slower than two separate units on the 30-qubit workload (4.06 vs 3.75 ms per
-    // dense-only pass), so it is opt-in
-    const char *penv = std::getenv("SVX_PAIR");
-    const bool pair_ok = penv && std::string(penv) == "on";
-    const char *oenv = std::getenv("SVX_TILE_ORDER");
-    const int torder = oenv ? std::atoi(oenv) : -2;   // default: each XCD streams a contiguous range of tiles
-    // runs of diagonal ops as one table each (complex64, single-buffer kernel; SVX_DIAG=off: never;
-    // SVX_FUSE=dense4 puts every op in a dense unit instead)
-    const char *genv = std::getenv("SVX_DIAG");
-    const bool diag_ok = !(genv && std::string(genv) == "off");
-    if (const char *cenv = std::getenv("SVX_FUSE_COST")) {
-        double c4 = fcfg.cost4, c5 = fcfg.cost5;
-        int wd = fcfg.width;
-        if (std::sscanf(cenv, "%lf,%lf,%d", &c4, &c5, &wd) >= 2) {
-            fcfg.cost4 = c4;
-            fcfg.cost5 = c5;
-            fcfg.width = std::min(5, std::max(1, wd));
-        }
-    }
+    const int torder = opt.tile_order == SVX_ORDER_IDENTITY ? 0 : -2;   // default: each XCD a contiguous range
+    const bool diag_ok = opt.diag != 0;
     const bool dma_ok = dtype == SVX_C64 && (k == 12 || k == 13) && nbits >= k + 9;
-    if ((pick == "dma" || pick == "pers" || pick == "dma5") && !dma_ok && !small)
-        return fail(SVX_E_ARG, "SVX_PASS_KERNEL=" + pick + " needs complex64, k in {12, 13} and >= 2^(k+9) amplitudes");
+    if (pick_dma && !dma_ok && !small)
+        return fail(SVX_E_ARG, "SVX_KERNEL_DMA needs complex64, k in {12, 13} and >= 2^(k+9) amplitudes");
     int Kv1 = 0;
     if (!small) {
         Kv1 = std::max(k, kMinK);
@@ -705,7 +801,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     std::vector<OpDev<T>> ops;
     std::vector<LaunchInfo> launches;
     std::vector<float> wmat;      // dense matrices, MFMA A-operand order
-    std::map<int, std::vector<double>> umat4;   // group index -> padded 16 x 16 matrix of a split-FP16 unit
     for (const RawPass &rp : seg.passes) {
         // classify
         std::vector<OpDev<T>> pops;
@@ -762,25 +857,21 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             cost += kOpCost[od.kind];
             pops.push_back(od);
         }
-        if (dtype == SVX_C64 && !small && diag_ok && fuse_mode != "dense4" && (pick == "auto" || pick == "v1")) {
+        if (dtype == SVX_C64 && !small && diag_ok && opt.fuse != SVX_FUSE_DENSE4 && !pick_dma) {
             diag_runs(pops, dense_of, dus, k, kDiagRunCost);
             for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
-        if (dtype == SVX_C64 && !small && fuse_mode != "off" &&
-            (pick == "auto" || pick == "v1" || pick == "dma" || pick == "dma5")) {
+        if (dtype == SVX_C64 && !small && opt.fuse != SVX_FUSE_OFF) {
             fuse_pass(pops, dense_of, dus, fcfg);
             for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
-        int mode;   // 0 small, 1 single-buffer, 2 DMA, 3 persistent
+        int mode;   // 0 small, 1 single-buffer, 2 DMA
         if (small) mode = 0;
-        else if (pick == "v1" || !dma_ok) mode = 1;
-        else if (has_dense) mode = (pick == "dma5" || pick == "dma") ? 2 : 1;   // dense groups: single-buffer kernel
-        else if (pick == "pers") mode = 3;
-        else if (pick == "dma" || pick == "dma5") mode = 2;
-        else mode = 1;   // light and heavy passes alike: the single-buffer kernel streams at ~6 TB/s
-        const bool dma5 = mode == 2 && k == 13 && pick == "dma5";
+        else if (pick_dma && dma_ok) mode = 2;
+        else mode = 1;   // light and heavy passes alike: the single-buffer kernel
+        (void)pick_single;
         const int K = mode == 0 ? nbits : (mode == 1 ? Kv1 : k);
-        const int R = mode == 2 ? (dma5 ? 5 : 4) : (mode == 3 ? (k == 12 ? 4 : 5) : kR);
+        const int R = mode == 2 ? 4 : kR;
         PassDev pd{};
         pd.k = k;
         pd.m = rp.m;
@@ -789,7 +880,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         pd.torder = torder;
         for (int i = 0; i < rp.m; ++i) pd.h[i] = pd.hs[i] = rp.h[i];
         std::sort(pd.hs, pd.hs + rp.m);
-        if (mode >= 2) {
+        if (mode == 2) {
             // uniform part of the DMA loader's element offsets: tile elements 128 j + (< 128)
             const int lowb = k - rp.m;
             for (int j = 0; j < (1 << k) / 128 && j < 64; ++j) {
@@ -808,27 +899,29 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         li.small = mode == 0;
         li.dense = has_dense;
         li.dma = mode == 2;
-        li.pers = mode == 3;
-        li.dma_variant = dma5 ? 100 : 0;
         li.K = K;
         li.pass_index = (int64_t)passes.size();
         li.empty = pd.nops == 0;
         li.ntiles = uint64_t(1) << (nbits - K);
-        // the tile order matters for HBM access once many workgroups run at a time; an explicit
-        // SVX_TILE_ORDER fixes it
-        li.tunable = mode == 1 && !oenv && li.ntiles >= 64;
+        // the tile order matters for HBM access once many workgroups run at a time; a fixed order
+        // option pins it
+        li.tunable = mode == 1 && opt.tile_order == SVX_ORDER_TUNABLE && li.ntiles >= 64;
         if (mode == 0) {
             li.grid = 1;
+            li.block = kNT;
             li.lds = 0;
-        } else if (mode == 3) {
-            li.grid = (uint32_t)std::min<uint64_t>(li.ntiles, 2 * (uint64_t)ds->caps.cu_count);
-            li.lds = pass_pers_lds_bytes(K, rp.m);
         } else if (mode == 2) {
             const uint64_t per_cu = K == 12 ? 2 : 1;   // LDS: two 2^K complex64 buffers per workgroup
             li.grid = (uint32_t)std::min<uint64_t>(li.ntiles, per_cu * (uint64_t)ds->caps.cu_count);
+            li.block = 64 + (K == 12 ? 256 : 512);
             li.lds = pass_dma_lds_bytes(K, rp.m);
         } else {
+            // one workgroup per tile: the grid in work-items must stay below 2^32 (HIP's limit)
+            if (li.ntiles * kNTD4 > 0xffffffffull)
+                return fail(SVX_E_CAPACITY, "2^" + std::to_string(nbits - K) + " workgroup tiles of 2^" + std::to_string(K) +
+                                                " amplitudes exceed the launch limit of 2^32 work-items");
             li.grid = (uint32_t)li.ntiles;
+            li.block = kNT;
             li.lds = pass_lds_bytes(K, rp.m, k, (int)sizeof(T) * 2, false, nbits);
         }
         launches.push_back(li);
@@ -960,7 +1053,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                 const int NRT = F == 5 ? 2 : 1;
                 while (wmat.size() % 4) wmat.push_back(0.f);   // 16-byte aligned operand blocks
                 gd.w_off = (int64_t)wmat.size();
-                if (dense_math == "f32" || (dense_math != "f16" && exact)) {
+                if (dense_f32 || exact) {
                     // FP32 A operand of v_mfma_f32_32x32x2f32: wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32]
                     gd.type = 1;
                     for (int rt = 0; rt < NRT; ++rt)
@@ -996,15 +1089,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
                     const size_t o = wmat.size();
                     wmat.resize(o + hb.size() / 2);
                     std::memcpy(wmat.data() + o, hb.data(), hb.size() * sizeof(uint16_t));
-                    if (F == 4) {   // kept for the paired form
-                        std::vector<double> um(size_t(2) * D * D);
-                        for (int r = 0; r < D; ++r)
-                            for (int c = 0; c < D; ++c) {
-                                um[2 * (size_t(r) * D + c)] = U(r, c, 0);
-                                um[2 * (size_t(r) * D + c) + 1] = U(r, c, 1);
-                            }
-                        umat4[(int)groups.size()] = std::move(um);
-                    }
                 }
                 // max-abs growth bound of the op: largest row sum of |U| (padding adds identity rows)
                 double rs_max = 0;
@@ -1092,52 +1176,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
         }
         pd.ngroups = (int)groups.size() - pd.group0;
         LaunchInfo &lr = launches.back();
-        // dense-only passes of 4-qubit units run on the 512-thread kernel (SVX_D4=off: never)
-        if (mode == 1 && has_dense && d4_mode != "off" && (K == 12 || K == 13) && pd.ngroups > 0 && k - rp.m >= 1) {
+        // dense-only passes of 4-qubit units run on the 512-thread kernel (dense_only = 0: never)
+        if (mode == 1 && has_dense && opt.dense_only && (K == 12 || K == 13) && pd.ngroups > 0 && k - rp.m >= 1) {
             bool all4 = true;
-            for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].f == 4;
+            for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].type != 6 && groups[g].f == 4;
             lr.d4 = all4;
-            // consecutive pre-scaled split-FP16 units on disjoint bits become pairs (one LDS round trip;
-            // the first of a pair gets type 5), their operands in the paired (B-operand) order:
-            // block (ks 2 + part), lane l, entry e = Q[16 ks + 8 (l / 32) + e][l % 32] with
-            // Q[(p, a)][(p', a')] = [[Re U, Im U], [-Im U, Re U]][p][p'] at U[a'][a]
-            if (all4 && pair_ok && pd.tscale > 0) {
-                auto pack_pair = [&](GroupDev &G, const std::vector<double> &um) {
-                    const double wsc = std::ldexp(1.0, G.wexp);
-                    while (wmat.size() % 4) wmat.push_back(0.f);
-                    G.w_off = (int64_t)wmat.size();
-                    std::vector<uint16_t> hb;
-                    hb.reserve(2 * 2 * 512);
-                    for (int ks = 0; ks < 2; ++ks)
-                        for (int part = 0; part < 2; ++part)
-                            for (int l = 0; l < 64; ++l)
-                                for (int e = 0; e < 8; ++e) {
-                                    const int a = 8 * (l / 32) + e, pp = (l % 32) / 16, ap = l % 16;
-                                    const double re = um[2 * (size_t(ap) * 16 + a)], im = um[2 * (size_t(ap) * 16 + a) + 1];
-                                    const double q = (ks == 0 ? (pp == 0 ? re : im) : (pp == 0 ? -im : re)) * wsc;
-                                    const uint16_t hh = f32_to_f16_bits((float)q);
-                                    hb.push_back(part == 0 ? hh : f32_to_f16_bits((float)(q - f16_bits_to_f32(hh))));
-                                }
-                    const size_t o = wmat.size();
-                    wmat.resize(o + hb.size() / 2);
-                    std::memcpy(wmat.data() + o, hb.data(), hb.size() * sizeof(uint16_t));
-                };
-                for (int g = pd.group0; g + 1 < (int)groups.size();) {
-                    GroupDev &A = groups[g], &B = groups[g + 1];
-                    bool ok = A.type == 4 && B.type == 4 && A.f == 4 && B.f == 4 && umat4.count(g) && umat4.count(g + 1);
-                    if (ok)
-                        for (int i = 0; i < 4; ++i)
-                            for (int j = 0; j < 4; ++j) ok = ok && A.dq[i] != B.dq[j];
-                    if (!ok) {
-                        ++g;
-                        continue;
-                    }
-                    pack_pair(A, umat4[g]);
-                    pack_pair(B, umat4[g + 1]);
-                    A.type = 5;
-                    g += 2;
-                }
-            }
+            if (all4) lr.block = kNTD4;
         }
         for (int g = pd.group0; g < (int)groups.size(); ++g) {
             const GroupDev &G = groups[g];
@@ -1152,7 +1196,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
             } else {
                 lr.stats[G.f == 5 ? 3 : 2] += 1;
                 lr.stats[4] += G.type == 1;
-                lr.stats[5] += G.type == 5;
             }
         }
         passes.push_back(pd);
@@ -1168,51 +1211,84 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, Prepared *pr) {
     pr->op_off = pr->group_off + al(groups.size() * sizeof(GroupDev));
     pr->w_off = pr->op_off + al(ops.size() * sizeof(OpDev<T>));
     const size_t total = std::max<size_t>(256, pr->w_off + al(wmat.size() * sizeof(float)));
-    std::vector<unsigned char> host(total, 0);
-    if (!passes.empty()) std::memcpy(host.data() + pr->pass_off, passes.data(), passes.size() * sizeof(PassDev));
-    if (!alt.empty()) std::memcpy(host.data() + pr->alt_off, alt.data(), alt.size() * sizeof(PassDev));
-    pr->alt.assign(passes.size(), 0);
+    blob->assign(total, 0);
+    if (!passes.empty()) std::memcpy(blob->data() + pr->pass_off, passes.data(), passes.size() * sizeof(PassDev));
+    if (!alt.empty()) std::memcpy(blob->data() + pr->alt_off, alt.data(), alt.size() * sizeof(PassDev));
+    pr->alt.reset(new std::atomic<uint8_t>[passes.size() + 1]);
+    for (size_t i = 0; i <= passes.size(); ++i) pr->alt[i].store(0);
     pr->torder = torder;
-    if (!groups.empty()) std::memcpy(host.data() + pr->group_off, groups.data(), groups.size() * sizeof(GroupDev));
-    if (!ops.empty()) std::memcpy(host.data() + pr->op_off, ops.data(), ops.size() * sizeof(OpDev<T>));
-    if (!wmat.empty()) std::memcpy(host.data() + pr->w_off, wmat.data(), wmat.size() * sizeof(float));
-    HIPCHK(hipMalloc(&pr->dev, total));
-    HIPCHK(hipMemcpy(pr->dev, host.data(), total, hipMemcpyHostToDevice));
-    g_prep_bytes += total;
+    if (!groups.empty()) std::memcpy(blob->data() + pr->group_off, groups.data(), groups.size() * sizeof(GroupDev));
+    if (!ops.empty()) std::memcpy(blob->data() + pr->op_off, ops.data(), ops.size() * sizeof(OpDev<T>));
+    if (!wmat.empty()) std::memcpy(blob->data() + pr->w_off, wmat.data(), wmat.size() * sizeof(float));
     pr->launches = launches;
     return 0;
 }
 
-int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle, Prepared **out) {
-    if (!w || nw <= 0) return fail(SVX_E_ARG, "empty QSV2 stream");
-    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "dtype must be SVX_C64 or SVX_C128");
-    DeviceState *ds = nullptr;
-    int rc = get_ds(&ds);
-    if (rc) return rc;
-    // the cache key includes the environment choices that change the descriptors
-    const char *kenv = std::getenv("SVX_PASS_KERNEL");
-    const char *denv = std::getenv("SVX_DENSE_MATH");
-    const char *fenv = std::getenv("SVX_FUSE");
-    const char *cenv = std::getenv("SVX_FUSE_COST");
-    const char *tenv = std::getenv("SVX_TILE_SCALE");
-    const char *d4env = std::getenv("SVX_D4");
-    const char *oenv = std::getenv("SVX_TILE_ORDER");
-    const char *penv = std::getenv("SVX_PAIR");
-    const char *genv = std::getenv("SVX_DIAG");
-    const std::string pick = std::string(kenv ? kenv : "auto") + "/" + (denv ? denv : "auto") + "/" +
-                             (fenv ? fenv : "auto") + "/" + (cenv ? cenv : "") + "/" + (tenv ? tenv : "") + "/" +
-                             (d4env ? d4env : "") + "/" + (oenv ? oenv : "") + "/" + (penv ? penv : "") + "/" +
-                             (genv ? genv : "");
-    const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
-                                        (uint64_t)ds->caps.device * 97u + std::hash<std::string>()(pick));
-    std::lock_guard<std::mutex> lk(g_prep_mu);
-    auto range = g_prep_index.equal_range(h);
+// Free the descriptors of unreferenced entries, oldest release first, while their bytes exceed the
+// cache limit; an entry whose last launch has not completed is kept for a later call.
+void evict_locked(DeviceState *ds) {
+    if (g_unref_bytes <= g_unref_limit) return;
+    std::vector<std::pair<uint64_t, int64_t>> cand;
+    for (size_t i = 0; i < g_prepared.size(); ++i) {
+        const Prepared *p = g_prepared[i].get();
+        if (p && p->refs == 0 && p->device == ds->caps.device) cand.push_back({p->lru, (int64_t)i + 1});
+    }
+    std::sort(cand.begin(), cand.end());
+    for (const auto &c : cand) {
+        if (g_unref_bytes <= g_unref_limit) break;
+        Prepared *p = g_prepared[c.second - 1].get();
+        if (p->last_use && hipEventQuery(p->last_use) != hipSuccess) continue;   // still in flight
+        auto range = g_prep_index.equal_range(p->key);
+        for (auto it = range.first; it != range.second; ++it)
+            if (it->second == c.second) {
+                g_prep_index.erase(it);
+                break;
+            }
+        ds->arena.release(p->dev, p->dev_bytes);
+        g_unref_bytes -= p->dev_bytes;
+        if (p->last_use) (void)hipEventDestroy(p->last_use);
+        g_prepared[c.second - 1].reset();
+    }
+}
+
+int64_t find_locked(uint64_t key, const int32_t *w, int64_t nw, int dtype, int nbits, int device, const Opts &o) {
+    auto range = g_prep_index.equal_range(key);
     for (auto it = range.first; it != range.second; ++it) {
         Prepared *p = g_prepared[it->second - 1].get();
-        if (p && p->dtype == dtype && p->nbits == nbits && p->device == ds->caps.device && p->pick == pick &&
-            (int64_t)p->words.size() == nw && std::memcmp(p->words.data(), w, nw * 4) == 0) {
-            *handle = it->second;
-            if (out) *out = p;
+        if (p && p->dtype == dtype && p->nbits == nbits && p->device == device &&
+            std::memcmp(&p->opts, &o, sizeof(Opts)) == 0 && (int64_t)p->words.size() == nw &&
+            std::memcmp(p->words.data(), w, nw * 4) == 0)
+            return it->second;
+    }
+    return 0;
+}
+
+void addref_locked(int64_t h) {
+    Prepared *p = g_prepared[h - 1].get();
+    if (p->refs++ == 0) g_unref_bytes -= p->dev_bytes;
+}
+
+// Look up or build the prepared form of a segment; the caller receives one reference.  The
+// build (classification, fusion search, operand layout) runs outside the registry lock.
+int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, const svx_options *opts, int64_t *handle,
+            Prepared **out) {
+    if (!w || nw <= 0) return fail(SVX_E_ARG, "empty QSV2 stream");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "dtype must be SVX_C64 or SVX_C128");
+    Opts o;
+    int rc = normalize_opts(opts, &o);
+    if (rc) return rc;
+    DeviceState *ds = nullptr;
+    rc = get_ds(&ds);
+    if (rc) return rc;
+    const uint64_t h = fnv1a(w, nw, (uint64_t)dtype * 1315423911u + (uint64_t)nbits * 2654435761u +
+                                        (uint64_t)ds->caps.device * 97u +
+                                        fnv1a(reinterpret_cast<const int32_t *>(&o), sizeof(Opts) / 4, 7));
+    {
+        std::lock_guard<std::mutex> lk(g_prep_mu);
+        if (int64_t hd = find_locked(h, w, nw, dtype, nbits, ds->caps.device, o)) {
+            addref_locked(hd);
+            *handle = hd;
+            if (out) *out = g_prepared[hd - 1].get();
             return 0;
         }
     }
@@ -1225,10 +1301,37 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     pr->nbits = nbits;
     pr->device = ds->caps.device;
     pr->words.assign(w, w + nw);
-    pr->pick = pick;
-    rc = dtype == SVX_C64 ? build_prepared<float>(seg, nbits, dtype, pr.get())
-                          : build_prepared<double>(seg, nbits, dtype, pr.get());
+    pr->opts = o;
+    pr->key = h;
+    std::vector<unsigned char> blob;
+    rc = dtype == SVX_C64 ? build_prepared<float>(seg, nbits, dtype, o, pr.get(), &blob)
+                          : build_prepared<double>(seg, nbits, dtype, o, pr.get(), &blob);
     if (rc) return rc;
+    void *dev = nullptr;
+    size_t got = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_prep_mu);
+        if (int64_t hd = find_locked(h, w, nw, dtype, nbits, ds->caps.device, o)) {   // built meanwhile
+            addref_locked(hd);
+            *handle = hd;
+            if (out) *out = g_prepared[hd - 1].get();
+            return 0;
+        }
+        evict_locked(ds);
+        rc = ds->arena.alloc(blob.size(), &dev, &got);
+        if (rc) return rc;
+    }
+    hipError_t e = hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pr->last_use, hipEventDisableTiming);
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    if (e != hipSuccess) {
+        ds->arena.release(dev, got);
+        if (pr->last_use) (void)hipEventDestroy(pr->last_use);
+        return hipfail(e, "descriptor upload");
+    }
+    pr->dev = dev;
+    pr->dev_bytes = got;
+    pr->refs = 1;
     g_prepared.push_back(std::move(pr));
     const int64_t hd = (int64_t)g_prepared.size();
     g_prep_index.insert({h, hd});
@@ -1237,23 +1340,43 @@ int prepare(const int32_t *w, int64_t nw, int dtype, int nbits, int64_t *handle,
     return 0;
 }
 
+int release(int64_t handle) {
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    Prepared *p = lookup_locked(handle);
+    if (!p) return fail(SVX_E_ARG, "unknown or released prepared-segment handle");
+    if (--p->refs == 0) {
+        g_unref_bytes += p->dev_bytes;
+        p->lru = ++g_lru_clock;
+        DeviceState *ds = nullptr;
+        std::lock_guard<std::mutex> lk2(g_mu);
+        auto it = g_dev.find(p->device);
+        if (it != g_dev.end()) ds = it->second.get();
+        if (ds) evict_locked(ds);
+    }
+    return 0;
+}
+
+// After a launch sequence that read a prepared segment's descriptors: remember its completion
+// (not under stream capture: a captured graph's owner keeps its handles referenced).
+void mark_use(Prepared *pr, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    (void)hipEventRecord(pr->last_use, s);
+}
+
 // one pass launch of a prepared segment (default or tuned tile order)
 hipError_t launch_one(void *state, const Prepared *pr, const LaunchInfo &li, hipStream_t s) {
     unsigned char *base = static_cast<unsigned char *>(pr->dev);
-    const PassDev *pd = reinterpret_cast<const PassDev *>(base + (pr->alt[li.pass_index] ? pr->alt_off : pr->pass_off)) +
-                        li.pass_index;
+    const bool alt = pr->alt[li.pass_index].load(std::memory_order_relaxed) != 0;
+    const PassDev *pd = reinterpret_cast<const PassDev *>(base + (alt ? pr->alt_off : pr->pass_off)) + li.pass_index;
     const GroupDev *gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
     const float *wd = li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr;
     if (li.d4)
         return launch_pass_d4(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
                               li.grid, li.lds, s);
-    if (li.pers)
-        return launch_pass_pers(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
-                                li.ntiles, li.grid, li.lds, s);
     if (li.dma)
-        return launch_pass_dma(li.K + li.dma_variant, state, pd, gd,
-                               reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd, li.ntiles, li.grid,
-                               li.lds, s);
+        return launch_pass_dma(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
+                               li.ntiles, li.grid, li.lds, s);
     if (pr->dtype == SVX_C64)
         return launch_pass_c64(li.K, li.small, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
                                wd, li.grid, li.lds, s);
@@ -1269,6 +1392,16 @@ int run_prepared(void *state, Prepared *pr, hipStream_t s) {
         const hipError_t e = launch_one(state, pr, li, s);
         if (e != hipSuccess) return hipfail(e, "pass kernel launch");
     }
+    mark_use(pr, s);
+    return 0;
+}
+
+// A referenced handle (error otherwise); the caller's reference keeps it alive after the lock.
+int get_prepared(int64_t handle, Prepared **out) {
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    Prepared *p = lookup_locked(handle);
+    if (!p) return fail(SVX_E_ARG, "unknown or released prepared-segment handle");
+    *out = p;
     return 0;
 }
 
@@ -1367,31 +1500,62 @@ int svx_init(int32_t device, svx_caps *out) {
     return 0;
 }
 
-int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle) {
+int svx_default_options(svx_options *out) {
+    if (!out) return fail(SVX_E_ARG, "null options pointer");
+    default_opts(out);
+    return 0;
+}
+
+int svx_prepare_ex(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, const svx_options *opts,
+                   int64_t *handle) {
     if (!handle) return fail(SVX_E_ARG, "null handle pointer");
     if (state_bits < 1 || state_bits > 40) return fail(SVX_E_ARG, "state_bits out of range");
-    return prepare(qsv2, n_words, dtype, state_bits, handle, nullptr);
+    return prepare(qsv2, n_words, dtype, state_bits, opts, handle, nullptr);
+}
+
+int svx_prepare(const int32_t *qsv2, int64_t n_words, int32_t dtype, int32_t state_bits, int64_t *handle) {
+    return svx_prepare_ex(qsv2, n_words, dtype, state_bits, nullptr, handle);
+}
+
+int svx_release_prepared(int64_t handle) { return release(handle); }
+
+int svx_cache_limit(int64_t limit_bytes, int64_t *held) {
+    if (limit_bytes < 0) return fail(SVX_E_ARG, "negative cache limit");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_prep_mu);
+    g_unref_limit = (size_t)limit_bytes;
+    evict_locked(ds);
+    if (held) {
+        size_t b = 0;
+        for (const auto &p : g_prepared)
+            if (p && p->device == ds->caps.device) b += p->dev_bytes;
+        *held = (int64_t)b;
+    }
+    return 0;
+}
+
+int svx_prepared_shape(int64_t handle, int32_t *dtype, int32_t *state_bits) {
+    Prepared *pr = nullptr;
+    int rc = get_prepared(handle, &pr);
+    if (rc) return rc;
+    if (dtype) *dtype = pr->dtype;
+    if (state_bits) *state_bits = pr->nbits;
+    return 0;
 }
 
 int svx_run_prepared(void *state, int64_t handle, void *stream) {
     Prepared *pr = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g_prep_mu);
-        if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
-            return fail(SVX_E_ARG, "unknown prepared-segment handle");
-        pr = g_prepared[handle - 1].get();
-    }
+    int rc = get_prepared(handle, &pr);
+    if (rc) return rc;
     return run_prepared(state, pr, (hipStream_t)stream);
 }
 
 int svx_tune_prepared(void *state, int64_t handle, int32_t reps, void *stream) {
     Prepared *pr = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g_prep_mu);
-        if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
-            return fail(SVX_E_ARG, "unknown prepared-segment handle");
-        pr = g_prepared[handle - 1].get();
-    }
+    int rc = get_prepared(handle, &pr);
+    if (rc) return rc;
     if (!state) return fail(SVX_E_ARG, "null state pointer");
     if (reinterpret_cast<uintptr_t>(state) & 15) return fail(SVX_E_ARG, "state must be 16-byte aligned");
     if (reps < 1 || reps > 64) return fail(SVX_E_ARG, "reps must be in 1..64");
@@ -1399,15 +1563,14 @@ int svx_tune_prepared(void *state, int64_t handle, int32_t reps, void *stream) {
     hipEvent_t ev[2];
     HIPCHK(hipEventCreate(&ev[0]));
     if (hipEventCreate(&ev[1]) != hipSuccess) {
-        hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[0]);
         return hipfail(hipErrorOutOfMemory, "event");
     }
-    int rc = 0;
     for (const LaunchInfo &li : pr->launches) {
         if (li.empty || !li.tunable) continue;
         float ms[2] = {0.f, 0.f};
         for (int c = 0; c < 2 && !rc; ++c) {
-            pr->alt[li.pass_index] = (int8_t)c;
+            pr->alt[li.pass_index].store((uint8_t)c);
             hipError_t e = launch_one(state, pr, li, s);   // untimed: first launch of this order
             if (e == hipSuccess) e = hipEventRecord(ev[0], s);
             for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_one(state, pr, li, s);
@@ -1417,11 +1580,12 @@ int svx_tune_prepared(void *state, int64_t handle, int32_t reps, void *stream) {
             if (e != hipSuccess) rc = hipfail(e, "tile-order timing");
         }
         // the identity order only when clearly faster (timing noise stays on the default)
-        pr->alt[li.pass_index] = (!rc && ms[1] < 0.97f * ms[0]) ? 1 : 0;
+        pr->alt[li.pass_index].store((!rc && ms[1] < 0.97f * ms[0]) ? 1 : 0);
         if (rc) break;
     }
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
+    mark_use(pr, s);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
     return rc;
 }
 
@@ -1440,13 +1604,16 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         size_t lds = 0;
     };
     std::map<int, Cls> cls;   // key: K * 4 + dense + 2 d4
+    std::vector<Prepared *> used;
     {
+        std::set<void *> seen;
+        for (int i = 0; i < n_items; ++i)
+            if (!seen.insert(states[i]).second)
+                return fail(SVX_E_ARG, "batch item " + std::to_string(i) + " repeats a state of an earlier item");
         std::lock_guard<std::mutex> lk(g_prep_mu);
         for (int i = 0; i < n_items; ++i) {
-            const int64_t h = handles[i];
-            if (h < 1 || h > (int64_t)g_prepared.size() || !g_prepared[h - 1])
-                return fail(SVX_E_ARG, "unknown prepared-segment handle in batch item " + std::to_string(i));
-            Prepared *pr = g_prepared[h - 1].get();
+            Prepared *pr = lookup_locked(handles[i]);
+            if (!pr) return fail(SVX_E_ARG, "unknown or released prepared-segment handle in batch item " + std::to_string(i));
             if (pr->dtype != SVX_C64) return fail(SVX_E_ARG, "batched launches run complex64 segments");
             if (pr->device != ds->caps.device) return fail(SVX_E_ARG, "batch item prepared on another device");
             if (passes[i] < 0 || passes[i] >= (int32_t)pr->launches.size())
@@ -1455,10 +1622,10 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
                 return fail(SVX_E_ARG, "batch states must be non-null and 16-byte aligned");
             const LaunchInfo &li = pr->launches[passes[i]];
             if (li.empty) continue;
-            if (li.small || li.dma || li.pers)
+            if (li.small || li.dma)
                 return fail(SVX_E_ARG, "batch item " + std::to_string(i) +
-                                           ": only passes of the single-buffer kernel batch (state >= 2^10 amplitudes, "
-                                           "SVX_PASS_KERNEL=auto|v1)");
+                                           ": only passes of the single-buffer kernels batch (state >= 2^10 amplitudes, "
+                                           "SVX_KERNEL_AUTO or SVX_KERNEL_SINGLE)");
             unsigned char *base = static_cast<unsigned char *>(pr->dev);
             BatchItem it{};
             it.st = states[i];
@@ -1471,6 +1638,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
             c.tiles += li.ntiles;
             c.lds = std::max(c.lds, li.lds);
             c.items.push_back(it);
+            used.push_back(pr);
         }
     }
     std::lock_guard<std::mutex> lk(ds->mu);
@@ -1481,7 +1649,9 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
     }
     for (auto &kv : cls) {
         Cls &c = kv.second;
-        if (c.tiles > 0xffffffffull) return fail(SVX_E_CAPACITY, "batch launch with more than 2^32 workgroups");
+        const uint64_t threads = (kv.first & 2) ? kNTD4 : kNT;
+        if (c.tiles * threads > 0xffffffffull)
+            return fail(SVX_E_CAPACITY, "batch launch beyond the launch limit of 2^32 work-items");
         const int slot = ds->batch_next;
         ds->batch_next = (ds->batch_next + 1) % DeviceState::kBatchSlots;
         if (ds->batch_used[slot]) HIPCHK(hipEventSynchronize(ds->batch_ev[slot]));
@@ -1495,14 +1665,16 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         HIPCHK(hipEventRecord(ds->batch_ev[slot], s));
         ds->batch_used[slot] = true;
     }
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    for (Prepared *pr : used) mark_use(pr, s);
     return 0;
 }
 
 int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches) {
-    std::lock_guard<std::mutex> lk(g_prep_mu);
-    if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
-        return fail(SVX_E_ARG, "unknown prepared-segment handle");
-    Prepared *pr = g_prepared[handle - 1].get();
+    Prepared *pr = nullptr;
+    int rc = get_prepared(handle, &pr);
+    if (rc) return rc;
     int nl = 0;
     for (auto &li : pr->launches) nl += li.empty ? 0 : 1;
     if (n_passes) *n_passes = (int32_t)pr->launches.size();
@@ -1511,18 +1683,16 @@ int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches) {
 }
 
 int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
-    std::lock_guard<std::mutex> lk(g_prep_mu);
-    if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
-        return fail(SVX_E_ARG, "unknown prepared-segment handle");
-    Prepared *pr = g_prepared[handle - 1].get();
+    Prepared *pr = nullptr;
+    int rc = get_prepared(handle, &pr);
+    if (rc) return rc;
     if (pass < 0 || pass >= (int32_t)pr->launches.size() || !name || cap < 1) return fail(SVX_E_ARG, "bad pass index");
     const LaunchInfo &li = pr->launches[pass];
     const char *T = pr->dtype == SVX_C64 ? "float" : "double";
     std::string k;
     if (li.empty) k = "none";
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
-    else if (li.pers) k = "k_pass_pers<" + std::to_string(li.K) + ">";
-    else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dma_variant ? ",5" : ",4") + (li.dense ? ",dense>" : ">");
+    else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dense ? ",dense>" : ">");
     else if (li.d4) k = "k_pass_d4<" + std::to_string(li.K) + ">";
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());
@@ -1530,18 +1700,17 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
 }
 
 int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out) {
-    std::lock_guard<std::mutex> lk(g_prep_mu);
-    if (handle < 1 || handle > (int64_t)g_prepared.size() || !g_prepared[handle - 1])
-        return fail(SVX_E_ARG, "unknown prepared-segment handle");
-    Prepared *pr = g_prepared[handle - 1].get();
+    Prepared *pr = nullptr;
+    int rc = get_prepared(handle, &pr);
+    if (rc) return rc;
     if (pass < 0 || pass >= (int32_t)pr->launches.size() || !out) return fail(SVX_E_ARG, "bad pass index");
     for (int i = 0; i < 7; ++i) out[i] = pr->launches[pass].stats[i];
-    out[7] = pr->alt[pr->launches[pass].pass_index] ? 0 : pr->torder;
+    out[7] = pr->alt[pr->launches[pass].pass_index].load() ? 0 : pr->torder;
     return 0;
 }
 
-int svx_run_segment(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t dtype, const int32_t *qsv2,
-                    int64_t n_words, void *stream) {
+int svx_run_segment_ex(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t dtype, const int32_t *qsv2,
+                       int64_t n_words, const svx_options *opts, void *stream) {
     const int L = ilog2_exact(n_slice_amps);
     if (L < 1) return fail(SVX_E_ARG, "n_slice_amps must be a power of two >= 2");
     if (n_slices < 1) return fail(SVX_E_ARG, "n_slices must be >= 1");
@@ -1550,24 +1719,27 @@ int svx_run_segment(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t
     if (qsv2[1] != L)
         return fail(SVX_E_ARG, "segment L = " + std::to_string(qsv2[1]) + " does not match the slice of 2^" + std::to_string(L));
     const int pb = ilog2_exact(n_slices);
-    if (pb >= 0) {
-        Prepared *pr = nullptr;
-        int64_t h = 0;
-        int rc = prepare(qsv2, n_words, dtype, L + pb, &h, &pr);
-        if (rc) return rc;
-        return run_prepared(state, pr, (hipStream_t)stream);
-    }
-    // a slice count that is not a power of two: one launch sequence per slice
     Prepared *pr = nullptr;
     int64_t h = 0;
-    int rc = prepare(qsv2, n_words, dtype, L, &h, &pr);
+    // a power-of-two slice count: one launch sequence over all slices; otherwise one per slice
+    int rc = prepare(qsv2, n_words, dtype, pb >= 0 ? L + pb : L, opts, &h, &pr);
     if (rc) return rc;
-    const size_t es = dtype == SVX_C64 ? 8 : 16;
-    for (int32_t p = 0; p < n_slices; ++p) {
-        rc = run_prepared(static_cast<unsigned char *>(state) + (size_t)p * n_slice_amps * es, pr, (hipStream_t)stream);
-        if (rc) return rc;
+    if (pb >= 0) {
+        rc = run_prepared(state, pr, (hipStream_t)stream);
+    } else {
+        const size_t es = dtype == SVX_C64 ? 8 : 16;
+        for (int32_t p = 0; p < n_slices && !rc; ++p)
+            rc = run_prepared(static_cast<unsigned char *>(state) + (size_t)p * n_slice_amps * es, pr, (hipStream_t)stream);
     }
-    return 0;
+    const std::string err = g_err;
+    (void)release(h);
+    if (rc) g_err = err;
+    return rc;
+}
+
+int svx_run_segment(void *state, int64_t n_slice_amps, int32_t n_slices, int32_t dtype, const int32_t *qsv2,
+                    int64_t n_words, void *stream) {
+    return svx_run_segment_ex(state, n_slice_amps, n_slices, dtype, qsv2, n_words, nullptr, stream);
 }
 
 int svx_engine(void *st, int64_t ins_off, void *stream) {

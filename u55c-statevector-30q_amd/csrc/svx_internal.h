@@ -7,6 +7,7 @@ namespace svx {
 
 constexpr int kR = 5;            // register bits per group: each thread holds 2^5 amplitudes
 constexpr int kNT = 256;         // threads per workgroup of the pass kernel (4 wavefronts)
+constexpr int kNTD4 = 512;       // threads per workgroup of the dense-only pass kernel
 constexpr int kMaxM = 8;         // gathered high qubits per pass (sub-block table 2^8 entries)
 constexpr int kMinK = 10;        // smallest workgroup tile of the main pass kernel
 constexpr int kMaxKc64 = 14;     // 2^14 complex64 = 128 KiB (+ padding) of LDS
@@ -49,9 +50,7 @@ struct GroupDev {
     int32_t bs[kR];              // b sorted ascending (slot deposit)
     int32_t type;                // 0: MUX2 register group; 1: dense op, FP32 MFMA; 2: dense op, small-state
                                  // kernel; 3: dense op, split-FP16 MFMA, per-column scaling; 4: the same
-                                 // on a tile pre-scaled by the pass kernel (PassDev::tscale); 5: first of
-                                 // a pair of 4-qubit type-4 units on disjoint bits applied in one LDS
-                                 // round trip (the next group is the second; operands in pair order);
+                                 // on a tile pre-scaled by the pass kernel (PassDev::tscale);
                                  // 6: diagonal table on its own (no MUX2 op beside it), 2^k complex64
                                  // phases at w_off indexed by the QSV2 tile index (complex64 k_pass)
     int32_t f;                   // dense: qubits of the op (type 1: 4 or 5 after padding)
@@ -104,12 +103,8 @@ hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, cons
                             const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t set_kernel_attributes();
 // double-buffered complex64 pass kernel (one QSV2 tile of 2^K per LDS buffer, K in {12, 13});
-// register groups of 4 (K = 12) or 5 (K = 13) tile bits
+// register groups of 4 tile bits
 size_t pass_dma_lds_bytes(int K, int m);
-// persistent single-buffer complex64 pass kernel (two workgroups per CU)
-size_t pass_pers_lds_bytes(int K, int m);
-hipError_t launch_pass_pers(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
-                            uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
                            const float *wd, uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
 size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits);

@@ -812,157 +812,6 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
 }
 
 // 4-qubit dense group only (the dense-only instance's register budget)
-// Two 4-qubit split-FP16 units A then B on disjoint tile bits, pre-scaled swizzled tile, one LDS round
-// trip.  Over the 8 unit bits and one further bit (the context), a wavefront step holds a 32 x 32 real
-// block: row (c, b) = 16 c + b (c the context bit, b the index over B's bits), columns (p, a), p the
-// re/im part, a the index over A's bits.  Unit A runs transposed, D = P Q with the data as the A
-// operand (P = [Re X^T | Im X^T], lane row (c, b), K = (p, a) in two 16-deep steps) and the weights as
-// the B operand (Q = [[Re U^T, Im U^T], [-Im U^T, Re U^T]]), giving D = [Re Y^T | Im Y^T] with lane l
-// holding column (p, a') = l % 32 and accumulator v row 8 (v / 4) + 4 (l / 32) + v % 4.  Swapping 16-lane
-// rows between accumulators v and v + 8 (contexts 0 and 1) and 32-lane halves between v and v + 4 turns
-// that into unit B's A operand -- lane row (c, a'), K = (p, b) -- with no LDS traffic; unit B's output
-// lane (p', b') then holds re or im of element (c, a', b') for accumulator v: row 16 c + a'; one more
-// row swap pairs re and im for the write-back.  wd holds, per unit, hi and lo B operands: block
-// (ks 2 + part), lane l, entry e = Q[16 ks + 8 (l / 32) + e][l % 32].  Each wavefront step reads and
-// writes 8 elements per lane.
-__device__ __forceinline__ void pair_split(const float (&x)[16], half8 (&hi)[2], half8 (&lo)[2]) {
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-        u4v hv, lv;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint32_t hh, ll;
-            split_pk(x[8 * ks + 2 * i], x[8 * ks + 2 * i + 1], hh, ll);
-            hv[i] = hh;
-            lv[i] = ll;
-        }
-        hi[ks] = __builtin_bit_cast(half8, hv);
-        lo[ks] = __builtin_bit_cast(half8, lv);
-    }
-}
-__device__ __forceinline__ f16v pair_mfma(const half8 (&ph)[2], const half8 (&pl)[2], const half8 (&qh)[2],
-                                          const half8 (&ql)[2]) {
-    f16v acc = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(pl[ks], qh[ks], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ph[ks], ql[ks], acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ph[ks], qh[ks], acc, 0, 0, 0);
-    return acc;
-}
-
-template <int K, int NWAVE>
-__device__ __forceinline__ void dense_pair_h(unsigned char *buf, const GroupDev &GA, const GroupDev &GB,
-                                             const float *__restrict__ wd, uint32_t wt) {
-    constexpr uint32_t NSTEP = 1u << (K - 9);   // contexts 2^(K-8), two per step
-    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, r = lane & 31u;
-    half8 qah[2], qal[2], qbh[2], qbl[2];
-    {
-        const half8 *wa = reinterpret_cast<const half8 *>(wd + GA.w_off);
-        const half8 *wb = reinterpret_cast<const half8 *>(wd + GB.w_off);
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            qah[ks] = wa[(ks * 2 + 0) * 64 + lane];
-            qal[ks] = wa[(ks * 2 + 1) * 64 + lane];
-            qbh[ks] = wb[(ks * 2 + 0) * 64 + lane];
-            qbl[ks] = wb[(ks * 2 + 1) * 64 + lane];
-        }
-    }
-    ColOff<4, true> ca, cb;
-    ca.init(GA);
-    cb.init(GB);
-    // the context bits: tile bits outside both units, ascending (deposit over the 8 unit bits)
-    int ub[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        ub[i] = GA.dq[i];
-        ub[4 + i] = GB.dq[i];
-    }
-#pragma unroll
-    for (int i = 1; i < 8; ++i)   // insertion sort of 8 uniform values
-#pragma unroll
-        for (int j = i; j > 0; --j)
-            if (ub[j - 1] > ub[j]) {
-                const int t = ub[j];
-                ub[j] = ub[j - 1];
-                ub[j - 1] = t;
-            }
-    auto depo = [&](uint32_t c) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) c = ins0_32(c, ub[i]);
-        return c;
-    };
-    const bool pairs = GA.dq[0] == 0;   // tile bit 0 = A's index bit 0: components a, a + 1 share a slot
-    // load side: lane row (c, b) = r, K slot a = 8 h + e
-    const uint32_t in_lane = cb(r & 15u) ^ swz(depo(r >> 4)) ^ ca(8u * h);
-    // store side (after the row swap): lane (c, b') = (r >> 4, r & 15), accumulator v -> a' = 8 ((v / 4) % 2)
-    // + 4 h + v % 4
-    const uint32_t st_lane = cb(r & 15u) ^ swz(depo(r >> 4)) ^ ca(4u * h);
-    for (uint32_t s = __builtin_amdgcn_readfirstlane(wave); s < NSTEP; s += NWAVE) {
-        const uint32_t ctxs = swz(depo(2u * s));
-        float x[16];   // x[8 p + e] = part p of X[a = 8 h + e][b] in context (r >> 4)
-        if (pairs) {
-#pragma unroll
-            for (int e = 0; e < 8; e += 2) {
-                const float4 v = *reinterpret_cast<const float4 *>(buf + (in_lane ^ ctxs ^ ca(e)));
-                x[e] = v.x;
-                x[8 + e] = v.y;
-                x[e + 1] = v.z;
-                x[8 + e + 1] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const C2<float> v = ldc(buf, in_lane ^ ctxs ^ ca(e));
-                x[e] = v.x;
-                x[8 + e] = v.y;
-            }
-        }
-        half8 ph[2], pl[2];
-        pair_split(x, ph, pl);
-        f16v acc = pair_mfma(ph, pl, qah, qal);
-        if (GA.wexp != 0) {
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[v] = ldexpf(acc[v], -GA.wexp);
-        }
-        // [Re Y^T | Im Y^T] -> unit B's A operand: contexts into 16-lane rows, then halves
-        float y[16];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(acc[v]), __float_as_uint(acc[v + 8]),
-                                                             false, false);
-            y[v] = __uint_as_float(sw[0]);       // part 0 (re), lane row 16 c + a'
-            y[v + 8] = __uint_as_float(sw[1]);   // part 1 (im)
-        }
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(y[8 * p + e]),
-                                                                 __float_as_uint(y[8 * p + 4 + e]), false, false);
-                y[8 * p + e] = __uint_as_float(sw[0]);       // K slot e: b = 8 h + e
-                y[8 * p + 4 + e] = __uint_as_float(sw[1]);   // K slot e + 4
-            }
-        pair_split(y, ph, pl);
-        acc = pair_mfma(ph, pl, qbh, qbl);
-        if (GB.wexp != 0) {
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[v] = ldexpf(acc[v], -GB.wexp);
-        }
-        // accumulators v (context 0) and v + 8 (context 1): swapping 16-lane rows puts re and im of
-        // one element in one lane -- lane (c, b') holds element (c, a'(v), b') -- one 8-byte write each
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(acc[v]), __float_as_uint(acc[v + 8]),
-                                                             false, false);
-            const uint32_t a = 8u * ((v / 4) % 2) + (v % 4);   // + 4 h: in st_lane
-            stc(buf, st_lane ^ ca(a) ^ ctxs, C2<float>{__uint_as_float(sw[0]), __uint_as_float(sw[1])});
-        }
-    }
-}
-
 template <int K, int NWAVE, bool SWZ = false>
 __device__ __forceinline__ void dense_group4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                              uint32_t wt) {
@@ -1118,14 +967,8 @@ __device__ __forceinline__ void pass_tile(C2<T> *__restrict__ st, const PassDev 
     for (int g = 0; g < ng; ++g) {
         const GroupDev &G = gd[g0 + g];
         if constexpr (D4 && sizeof(T) == 4) {
-            // dense-only passes: no MUX2 path compiled in; a pair head (type 5) applies itself and the
-            // next group in one LDS round trip
-            if (G.type == 5) {
-                dense_pair_h<K, NT / 64>(smem, G, gd[g0 + g + 1], wd, tid);
-                ++g;
-            } else {
-                dense_group4<K, NT / 64, true>(smem, G, wd, tid);
-            }
+            // dense-only passes: no MUX2 path compiled in
+            dense_group4<K, NT / 64, true>(smem, G, wd, tid);
             __syncthreads();
             continue;
         } else if constexpr (DN && sizeof(T) == 4) {
@@ -1212,7 +1055,6 @@ __global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass(C2<T> *__r
 
 // Dense-only passes with 4-qubit units: 512 threads and at most 128 VGPRs (90 used), so two workgroups
 // give four waves per SIMD to hide one wave's LDS -> split -> MFMA -> LDS chain behind the others.
-constexpr int kNTD4 = 512;
 template <int K>
 __global__ __launch_bounds__(kNTD4, 4) void k_pass_d4(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                                       const GroupDev *__restrict__ gd,
@@ -1379,105 +1221,6 @@ __global__ __launch_bounds__(64 + NW) void k_pass_dma(C2<float> *__restrict__ st
     }
 }
 
-// Persistent single-buffer variant: every wave loads (16-byte global_load_lds), computes and
-// stores each tile in turn; two workgroups share a CU so one computes while the other streams.
-template <int K, int R, int NT>
-__global__ __launch_bounds__(NT, 2) void k_pass_pers(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
-                                                  const GroupDev *__restrict__ gd,
-                                                  const OpDev<float> *__restrict__ od, uint64_t ntiles) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    constexpr uint32_t NE = 1u << K, NS = NE / 2, NJ = NS / 64, NWAVE = NT / 64;
-    constexpr int NR = 1 << R;
-    constexpr uint32_t NSLOT = NE >> R;
-    static_assert(NJ % NWAVE == 0 && (NWAVE & 3) == 0, "each wave issues every 4th DMA instruction class");
-    unsigned char *buf = smem;
-    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + NE * 8);
-    const int m = pd->m, lowb = K - m;
-    const uint32_t lowmask = (1u << lowb) - 1u;
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-    for (uint32_t q = tid; q < (1u << m); q += NT) {
-        uint64_t o = 0;
-        for (int i = 0; i < m; ++i)
-            if ((q >> i) & 1u) o |= uint64_t(1) << pd->h[i];
-        sbo[q] = o;
-    }
-    lds_barrier();
-    // this wave issues DMA instructions j = wave + NWAVE i, all with j & 3 == wave & 3
-    const uint32_t hq = (4u * (wave & 3u) + lane / 16u) & 15u;
-    const uint32_t eo = 2u * (lane ^ hq);
-    const uint64_t vlo = (lowb < 7 ? sbo[eo >> lowb] : 0) | (eo & lowmask);
-    const unsigned char *sbytes = reinterpret_cast<const unsigned char *>(st);
-    float *sf = reinterpret_cast<float *>(st);
-    const int ng = pd->ngroups, g0 = pd->group0;
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        uint64_t tb = t << lowb;
-        for (int i = 0; i < m; ++i) tb = ins0_64(tb, pd->hs[i]);
-        const unsigned char *tile = sbytes + tb * sizeof(C2<float>);
-#pragma unroll
-        for (uint32_t i = 0; i < NJ / NWAVE; ++i) {
-            const uint32_t j = wave + NWAVE * i;
-            __builtin_amdgcn_global_load_lds(tile + (pd->dma_u[j] | vlo) * sizeof(C2<float>),
-                                             (lds_void_t *)(buf + j * 1024u), 16, 0, 0);
-        }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        lds_barrier();
-        for (int g = 0; g < ng; ++g) {
-            const GroupDev &Gd = gd[g0 + g];
-            uint32_t boff[NR];
-            boff[0] = 0;
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const uint32_t bq = swz(1u << Gd.b[q]);
-#pragma unroll
-                for (int r = (1 << q); r < (2 << q); ++r) boff[r] = boff[r - (1 << q)] ^ bq;
-            }
-            const int nops = Gd.nops, o0 = Gd.op0;
-#pragma unroll 1
-            for (uint32_t sl = tid; sl < NSLOT; sl += NT) {
-                uint32_t bidx = sl;
-#pragma unroll
-                for (int q = 0; q < R; ++q) bidx = ins0_32(bidx, Gd.bs[q]);
-                const uint32_t bb = swz(bidx);
-                C2<float> a[NR];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) a[r] = *reinterpret_cast<const C2<float> *>(buf + (bb ^ boff[r]));
-#pragma unroll 1
-                for (int o = 0; o < nops; ++o) {
-                    const OpDev<float> &op = od[o0 + o];
-                    const bool ts = (bidx & op.cmt) == op.cmt;
-                    apply_op<float, R>(a, op, ts);
-                }
-#pragma unroll
-                for (int r = 0; r < NR; ++r) *reinterpret_cast<C2<float> *>(buf + (bb ^ boff[r])) = a[r];
-            }
-            lds_barrier();
-        }
-#pragma unroll 4
-        for (uint32_t S = tid; S < NS; S += NT) {
-            const uint32_t e = 2u * (S ^ ((S >> 4) & 15u));
-            const uint64_t gi = tb | sbo[e >> lowb] | (e & lowmask);
-            const float4 v = *reinterpret_cast<const float4 *>(buf + 16u * S);
-            *reinterpret_cast<float4 *>(sf + 2 * gi) = v;
-        }
-        lds_barrier();
-    }
-}
-
-size_t pass_pers_lds_bytes(int K, int m) {
-    return (size_t(1) << K) * 8 + (size_t(1) << m) * sizeof(uint64_t);
-}
-
-hipError_t launch_pass_pers(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
-                            uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s) {
-    C2<float> *p = reinterpret_cast<C2<float> *>(st);
-    switch (K) {
-    case 12: hipLaunchKernelGGL((k_pass_pers<12, 4, 256>), dim3(grid), dim3(256), lds, s, p, pd, gd, od, ntiles); break;
-    case 13: hipLaunchKernelGGL((k_pass_pers<13, 5, 256>), dim3(grid), dim3(256), lds, s, p, pd, gd, od, ntiles); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 size_t pass_dma_lds_bytes(int K, int m) {
     return (size_t(2) << K) * 8 + (size_t(1) << m) * sizeof(uint64_t);
 }
@@ -1489,7 +1232,6 @@ hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *g
         switch (K) {
         case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
         case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512, true>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, wd, ntiles); break;
-        case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -1497,7 +1239,6 @@ hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *g
     switch (K) {
     case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, false>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
     case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512, false>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, wd, ntiles); break;
-    case 13 + 100: hipLaunchKernelGGL((k_pass_dma<13, 5, 256, false>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2032,9 +1773,7 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass<double, 10, false>)) SVX_ATTR((k_pass<double, 11, false>)) SVX_ATTR((k_pass<double, 12, false>))
     SVX_ATTR((k_pass<double, 13, false>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
     SVX_ATTR((k_pass_dma<12, 4, 256, false>)) SVX_ATTR((k_pass_dma<13, 4, 512, false>))
-    SVX_ATTR((k_pass_dma<13, 5, 256, false>)) SVX_ATTR((k_pass_dma<12, 4, 256, true>))
-    SVX_ATTR((k_pass_dma<13, 5, 256, true>)) SVX_ATTR((k_pass_dma<13, 4, 512, true>))
-    SVX_ATTR((k_pass_pers<12, 4, 256>)) SVX_ATTR((k_pass_pers<13, 5, 256>))
+    SVX_ATTR((k_pass_dma<12, 4, 256, true>)) SVX_ATTR((k_pass_dma<13, 4, 512, true>))
 #undef SVX_ATTR
     return e;
 }

@@ -7,8 +7,10 @@ host/sv/model.py:9; ValueError for bad arguments; RuntimeError for capacity / ru
 host/sv/runtime.py:220-221).  The library must be built (python csrc/build.py); there is no
 fallback path.
 """
+import contextlib
 import ctypes
 import os
+import threading
 
 import numpy as np
 
@@ -18,10 +20,72 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libsvx.so")
 C64, C128 = 0, 1
 SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -1, -2, -3, -4, -5, -6
 
-SYMBOLS = ["svx_init", "svx_run_segment", "svx_engine", "svx_prepare", "svx_run_prepared",
-           "svx_tune_prepared", "svx_prepared_info", "svx_prepared_kernel", "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state", "svx_product_state",
-           "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync", "svx_last_error",
-           "svx_version"]
+SYMBOLS = ["svx_init", "svx_run_segment", "svx_run_segment_ex", "svx_engine", "svx_default_options",
+           "svx_prepare", "svx_prepare_ex", "svx_release_prepared", "svx_cache_limit", "svx_prepared_shape",
+           "svx_run_prepared", "svx_tune_prepared", "svx_prepared_info", "svx_prepared_kernel",
+           "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state",
+           "svx_product_state", "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync",
+           "svx_last_error", "svx_version"]
+
+FUSE = {"auto": 0, "on": 0, "off": 1, "dense4": 2}
+DENSE_MATH = {"f32": 0, "split16": 1}
+KERNEL = {"auto": 0, "single": 1, "dma": 2}
+TILE_ORDER = {"tunable": 0, "identity": 1, "xcd": 2}
+MUX_ENGINE = {"auto": 0, "valu": 1}
+
+
+class Options(ctypes.Structure):
+    """svx_options: how a segment is prepared (include/svx.h)."""
+    _fields_ = [("size", ctypes.c_int32), ("fuse", ctypes.c_int32), ("dense_math", ctypes.c_int32),
+                ("kernel", ctypes.c_int32), ("diag_tables", ctypes.c_int32), ("dense_only", ctypes.c_int32),
+                ("tile_scale", ctypes.c_int32), ("tile_order", ctypes.c_int32), ("mux_engine", ctypes.c_int32),
+                ("fuse_width", ctypes.c_int32), ("fuse_cost4", ctypes.c_float), ("fuse_cost5", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 4)]
+
+
+def make_options(fuse="auto", dense_math="f32", kernel="auto", diag_tables=True, dense_only=True,
+                 tile_scale=True, tile_order="tunable", mux_engine="auto", fuse_width=0, fuse_cost4=0.0,
+                 fuse_cost5=0.0):
+    """An Options value from names (defaults = the library's defaults: FP32 everywhere)."""
+    o = Options()
+    o.size = ctypes.sizeof(Options)
+    o.fuse = FUSE[fuse]
+    o.dense_math = DENSE_MATH[dense_math]
+    o.kernel = KERNEL[kernel]
+    o.diag_tables = int(bool(diag_tables))
+    o.dense_only = int(bool(dense_only))
+    o.tile_scale = int(bool(tile_scale))
+    o.tile_order = TILE_ORDER[tile_order]
+    o.mux_engine = MUX_ENGINE[mux_engine]
+    o.fuse_width = int(fuse_width)
+    o.fuse_cost4 = float(fuse_cost4)
+    o.fuse_cost5 = float(fuse_cost5)
+    return o
+
+
+_tls = threading.local()
+
+
+def current_options():
+    """The calling thread's options (innermost `options(...)` block), or None for the defaults."""
+    st = getattr(_tls, "stack", None)
+    return st[-1][1] if st else None
+
+
+@contextlib.contextmanager
+def options(**kw):
+    """Thread-local preparation options for prepare / run_segment calls inside the block, layered
+    over the enclosing block's settings (no environment variables, no process-wide state)."""
+    st = getattr(_tls, "stack", None)
+    if st is None:
+        st = _tls.stack = []
+    merged = dict(st[-1][0]) if st else {}
+    merged.update(kw)
+    st.append((merged, make_options(**merged)))
+    try:
+        yield st[-1][1]
+    finally:
+        st.pop()
 
 
 class SvxCaps(ctypes.Structure):
@@ -40,8 +104,14 @@ _PD = ctypes.POINTER(ctypes.c_double)
 _SIGS = {
     "svx_init": ([_I32, ctypes.POINTER(SvxCaps)], _I32),
     "svx_run_segment": ([_VP, _I64, _I32, _I32, _PI32, _I64, _VP], _I32),
+    "svx_run_segment_ex": ([_VP, _I64, _I32, _I32, _PI32, _I64, ctypes.POINTER(Options), _VP], _I32),
     "svx_engine": ([_VP, _I64, _VP], _I32),
+    "svx_default_options": ([ctypes.POINTER(Options)], _I32),
     "svx_prepare": ([_PI32, _I64, _I32, _I32, ctypes.POINTER(_I64)], _I32),
+    "svx_prepare_ex": ([_PI32, _I64, _I32, _I32, ctypes.POINTER(Options), ctypes.POINTER(_I64)], _I32),
+    "svx_release_prepared": ([_I64], _I32),
+    "svx_cache_limit": ([_I64, ctypes.POINTER(_I64)], _I32),
+    "svx_prepared_shape": ([_I64, _PI32, _PI32], _I32),
     "svx_run_prepared": ([_VP, _I64, _VP], _I32),
     "svx_tune_prepared": ([_VP, _I64, _I32, _VP], _I32),
     "svx_prepared_info": ([_I64, _PI32, _PI32], _I32),
@@ -135,12 +205,19 @@ def version():
     return load().svx_version().decode()
 
 
-def run_segment(state, words, n_slices=1, stream=None):
-    """svx_run_segment: every 2^L slice of `state` (n_slices of them) runs the QSV2 segment."""
+def _opts_ptr(opts):
+    o = opts if opts is not None else current_options()
+    return None if o is None else ctypes.byref(o)
+
+
+def run_segment(state, words, n_slices=1, stream=None, opts=None):
+    """svx_run_segment: every 2^L slice of `state` (n_slices of them) runs the QSV2 segment, with
+    `opts` (an Options) or the thread's current options."""
     lib = load()
     w, wp = _i32(words)
     n_slice = state.numel() // n_slices
-    _check(lib.svx_run_segment(_dev(state), n_slice, n_slices, dtype_code(state), wp, w.size, _stream(stream)))
+    _check(lib.svx_run_segment_ex(_dev(state), n_slice, n_slices, dtype_code(state), wp, w.size, _opts_ptr(opts),
+                                  _stream(stream)))
 
 
 def engine(buf_u8, ins_off, stream=None):
@@ -150,38 +227,52 @@ def engine(buf_u8, ins_off, stream=None):
     _check(lib.svx_engine(_dev(buf_u8), int(ins_off), _stream(stream)))
 
 
-class fusion:
-    """Context manager selecting the library's pass fusion for segments prepared inside it
-    (SVX_FUSE: "auto" clusters MUX2 ops into dense units, "off" runs every op as MUX2)."""
-
-    def __init__(self, mode):
-        if mode not in ("auto", "on", "off"):
-            raise ValueError("fusion mode must be auto, on or off")
-        self.mode = mode
-
-    def __enter__(self):
-        self.prev = os.environ.get("SVX_FUSE")
-        os.environ["SVX_FUSE"] = self.mode
-        return self
-
-    def __exit__(self, *exc):
-        if self.prev is None:
-            os.environ.pop("SVX_FUSE", None)
-        else:
-            os.environ["SVX_FUSE"] = self.prev
-        return False
+def fusion(mode):
+    """Options block selecting the library's pass fusion ("auto" clusters MUX2 ops into dense units
+    where that is cheaper, "off" runs every op as MUX2)."""
+    if mode not in ("auto", "on", "off", "dense4"):
+        raise ValueError("fusion mode must be auto, on, off or dense4")
+    return options(fuse=mode)
 
 
-def prepare(words, dtype, state_bits):
+def prepare(words, dtype, state_bits, opts=None):
+    """svx_prepare_ex: a handle holding one reference (give it back with release_prepared)."""
     lib = load()
     w, wp = _i32(words)
     h = ctypes.c_int64(0)
-    _check(lib.svx_prepare(wp, w.size, dtype, state_bits, ctypes.byref(h)))
+    _check(lib.svx_prepare_ex(wp, w.size, dtype, state_bits, _opts_ptr(opts), ctypes.byref(h)))
     return h.value
+
+
+def release_prepared(handle):
+    _check(load().svx_release_prepared(int(handle)))
+
+
+def cache_limit(limit_bytes):
+    """Set the byte limit of the cache of unreferenced descriptors; returns the descriptor bytes held."""
+    held = ctypes.c_int64(0)
+    _check(load().svx_cache_limit(int(limit_bytes), ctypes.byref(held)))
+    return held.value
+
+
+def prepared_shape(handle):
+    lib = load()
+    dt, nb = ctypes.c_int32(0), ctypes.c_int32(0)
+    _check(lib.svx_prepared_shape(int(handle), ctypes.byref(dt), ctypes.byref(nb)))
+    return dt.value, nb.value
+
+
+def _check_state(state, handle):
+    dt, nb = prepared_shape(handle)
+    if dtype_code(state) != dt:
+        raise ValueError(f"state dtype {state.dtype} does not match the prepared segment's dtype")
+    if state.numel() != 1 << nb:
+        raise ValueError(f"state holds {state.numel()} amplitudes; the segment was prepared for 2^{nb}")
 
 
 def run_prepared(state, handle, stream=None):
     lib = load()
+    _check_state(state, handle)
     _check(lib.svx_run_prepared(_dev(state), handle, _stream(stream)))
 
 
@@ -190,6 +281,7 @@ def tune_prepared(state, handle, reps=2, stream=None):
     `state` (synchronous; the passes are applied 2 (1 + reps) times to it, so use a state whose
     contents may change).  Results of later runs stay bit-identical."""
     lib = load()
+    _check_state(state, handle)
     _check(lib.svx_tune_prepared(_dev(state), handle, int(reps), _stream(stream)))
 
 
@@ -214,9 +306,12 @@ def run_batch(states, handles, passes, stream=None):
     n = len(states)
     if not (len(handles) == len(passes) == n):
         raise ValueError("states, handles and passes must have the same length")
-    for t in states:
+    for t, h in zip(states, handles):
         if dtype_code(t) != C64:
             raise ValueError("batched launches run complex64 states")
+        _check_state(t, h)
+    if len({t.data_ptr() for t in states}) != n:
+        raise ValueError("batch items must touch distinct states")
     sp = (ctypes.c_void_p * n)(*[_dev(t) for t in states])
     hp = (ctypes.c_int64 * n)(*[int(h) for h in handles])
     pp = (ctypes.c_int32 * n)(*[int(p) for p in passes])
@@ -225,14 +320,14 @@ def run_batch(states, handles, passes, stream=None):
 
 def prepared_pass_stats(handle, pass_index):
     """Work of one pass after classification and fusion: MUX2 groups, MUX2 ops, dense units on 4
-    and 5 qubits, dense units on the exact FP32 path, pairs of 4-qubit units sharing an LDS round trip,
-    diagonal tables (runs of diagonal ops applied as one table of phases), and the workgroup-to-tile
-    order the pass runs with (-2: XCD-contiguous, 0: identity)."""
+    and 5 qubits, dense units on the FP32 matrix cores (all of them in the default FP32 mode),
+    MUX2 ops on the matrix cores, diagonal tables (runs of diagonal ops applied as one table of
+    phases), and the workgroup-to-tile order the pass runs with (-2: XCD-contiguous, 0: identity)."""
     lib = load()
     out = (ctypes.c_int32 * 8)()
     _check(lib.svx_prepared_pass_stats(handle, pass_index, out))
     return dict(mux_groups=out[0], mux_ops=out[1], dense4=out[2], dense5=out[3], dense_exact=out[4],
-                dense_pairs=out[5], diag_tables=out[6], tile_order=out[7])
+                mux_mfma=out[5], diag_tables=out[6], tile_order=out[7])
 
 
 def permute(dst, src, perm, stream=None):

@@ -130,6 +130,8 @@ class Autotuner:
                 e1.record(stream)
                 torch.cuda.synchronize()
                 table[f"k{k}/{form}"] = round(e0.elapsed_time(e1), 3)
+                for h in hs:
+                    libsvx.release_prepared(h)
         best = min(table, key=table.get)
         kk, form = best.split("/")
         rec = dict(k=int(kk[1:]), form=form, ms=table[best], table=table,

@@ -138,6 +138,13 @@ class SVServer:
                 self.active.append(req)
 
     @staticmethod
+    def _release(req):
+        """Give back the request's prepared handles (their descriptors become evictable)."""
+        for h in {h for h, _ in req.steps}:
+            libsvx.release_prepared(h)
+        req.steps = []
+
+    @staticmethod
     def _segments(req):
         seen = []
         for h, p in req.steps:
@@ -191,8 +198,9 @@ class SVServer:
                 r.t_done = self.now()
                 if not self.keep_states:
                     r.state = None
-                    r.graph = None
                     self.in_use -= r.footprint
+                r.graph = None            # replayed once; its handles are given back below
+                self._release(r)
                 self.done.append(r)
             else:
                 left.append(r)
