@@ -50,7 +50,8 @@ extern "C" {
 #define SVX_ORDER_TUNABLE 0  /* XCD-contiguous tile order; svx_tune_prepared may switch passes to identity */
 #define SVX_ORDER_IDENTITY 1 /* fixed identity workgroup -> tile order */
 #define SVX_ORDER_XCD 2      /* fixed XCD-contiguous order (not tunable) */
-#define SVX_MUX_AUTO 0       /* uncontrolled 2x2 ops share FP32 vector units and FP32 matrix cores */
+#define SVX_MUX_AUTO 0       /* general uncontrolled 2x2 ops on the FP32 matrix cores (v_mfma_f32_4x4x1_16b_f32,
+                                FP32 products, one FMA rounding per step), the rest on the FP32 vector units */
 #define SVX_MUX_VALU 1       /* every MUX2 op on the FP32 vector units */
 
 typedef struct svx_options {

@@ -22,9 +22,11 @@ C_BOUND = 4.0
 
 
 def seg_nops(words):
-    """Number of op records (MUX2 and dense) in a QSV2 segment."""
+    """Ops of a QSV2 segment in the bound's units: a MUX2 record counts 1, a dense record on f qubits
+    counts f (it stands for at least f one-qubit ops, and each of its outputs sums 2^f products)."""
+    from oracle.isa import OP_DENSE
     _, _, passes = model.decode(np.asarray(words))
-    return sum(len(p[-1]) for p in passes)
+    return sum(len(op[1]) if op[0] == OP_DENSE else 1 for p in passes for op in p[-1])
 
 
 def amp_error_ratio(ref, got, nops):

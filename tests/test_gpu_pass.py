@@ -227,9 +227,9 @@ def test_prepared_replay_is_identical(gpu):
 @pytest.mark.parametrize("n,k,max_m", [(21, 12, 4), (21, 12, 7), (22, 13, 7), (22, 13, 8)])
 def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m):
     """The double-buffered kernel (one QSV2 tile per LDS buffer, loader wave + worker waves) and
-    the single-buffer kernel apply the same ops in the same order with the same arithmetic (every
-    op on the vector units): their results are bit-identical, and both match the complex128
-    golden model."""
+    the FP32 MUX2 kernel (single-buffer, two register buffers, ops regrouped over their
+    commutation DAG) both match the complex128 golden model to the FP32 bound, and each other to
+    twice that bound (different FMA orders, so not bit-identical)."""
     rng = np.random.default_rng(n * 100 + k + max_m)
     seg = random_segment(rng, n, k, n_passes=2, max_ops=10, max_m=max_m)
     s0 = _rand_state(rng, n, np.complex64)
@@ -239,7 +239,7 @@ def test_dma_kernel_matches_oracle_and_v1(gpu, n, k, max_m):
         with _opts(fuse="off", diag_tables=False, mux_engine="valu", kernel=pick):
             got[pick] = run_hip(s0, seg)
         _check_c64(ref, got[pick], (n, k, max_m, pick), seg)
-    assert np.array_equal(got["dma"], got["single"])
+    check_c64(got["dma"].astype(np.complex128), got["single"], (n, k, max_m, "dma vs single"), seg_nops(seg), c=8.0)
 
 
 FUSE_SETTINGS = {"off": dict(fuse="off"), "auto": dict(fuse="auto"),

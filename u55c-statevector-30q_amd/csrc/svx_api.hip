@@ -349,6 +349,7 @@ struct LaunchInfo {
     bool dense;           // pass holds dense (fused-form) groups or tables: the dense instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
     bool tunable;         // single-buffer / dense-only launch whose tile order svx_tune_prepared may switch
+    bool mux;             // the FP32 MUX2 kernel k_mux (complex64 passes of MUX2 ops and diagonal tables)
     int32_t stats[7];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32),
                           // MUX2 ops on the matrix cores, diagonal tables
 };
@@ -754,6 +755,190 @@ void diag_runs(std::vector<OpDev<T>> &pops, std::vector<int> &dense_of, std::vec
     dense_of.swap(out_dense);
 }
 
+// Register groups of the FP32 MUX2 kernel.  Ops are list-scheduled over their commutation DAG:
+// two ops commute when neither one's mixing bit (the target of a non-diagonal op; none for a
+// diagonal op or table) lies in the other's support (target, controls, table bits), so only
+// commuting ops change order and the pass's product of ops is unchanged.  A group collects ready
+// ops while their register targets fit R bits, preferring ops whose target is already a register
+// bit; a controlled phase diag(1, e) acts symmetrically on all its bits, so any of them may be its
+// register target (the others become controls).  Each op becomes an M_* record with its
+// coefficients in the form the kernel consumes.
+template <typename T, typename PutTable>
+int mux_groups(const std::vector<OpDev<T>> &pops, const std::vector<int> &dense_of, const std::vector<DenseUnit> &dus,
+               int K, int R, bool mfma, std::vector<GroupDev> &groups, std::vector<OpDev<T>> &ops,
+               PutTable &&put_table_regs, double &log2_growth, int32_t *n_mfma) {
+    const int n = (int)pops.size();
+    std::vector<uint32_t> sup(n), mix(n), tc(n);
+    for (int i = 0; i < n; ++i) {
+        if (dense_of[i] >= 0) {   // diagonal table
+            sup[i] = dus[dense_of[i]].dmask;
+            mix[i] = 0;
+            tc[i] = 0;
+            continue;
+        }
+        const OpDev<T> &o = pops[i];
+        sup[i] = (1u << o.t) | (uint32_t)o.cmask;
+        const bool diagonal = is_diag(o.m0) && (o.cmask == 0 || is_diag(o.m1));
+        mix[i] = diagonal ? 0u : (1u << o.t);
+        const bool cphase = o.kind == K_CPHASE || o.kind == K_CPHASE1;
+        tc[i] = cphase ? sup[i] : (1u << o.t);
+    }
+    std::vector<uint64_t> pred(n, 0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < j; ++i)
+            if ((mix[i] & sup[j]) || (mix[j] & sup[i])) pred[j] |= uint64_t(1) << i;
+    uint64_t done = 0;
+    const uint64_t all = n == 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
+    int n_full = 0;
+    while (done != all) {
+        uint32_t S = 0;
+        int nS = 0;
+        std::vector<std::pair<int, int>> mem;   // (op, register target tile bit; -1 for a table)
+        while (true) {
+            int best = -1, bestcost = 99, bestbit = -1;
+            for (int i = 0; i < n; ++i) {
+                if (((done >> i) & 1u) || (pred[i] & ~done)) continue;
+                int cost, bit;
+                if (tc[i] == 0) {
+                    cost = 0;
+                    bit = -1;
+                } else if (tc[i] & S) {
+                    cost = 0;
+                    bit = __builtin_ctz(tc[i] & S);
+                    if ((1u << pops[i].t) & S) bit = pops[i].t;   // keep the record's own target when possible
+                } else if (nS < R) {
+                    cost = 1;
+                    bit = pops[i].t;
+                } else {
+                    continue;
+                }
+                if (cost < bestcost) {
+                    best = i;
+                    bestcost = cost;
+                    bestbit = bit;
+                }
+            }
+            if (best < 0) break;
+            done |= uint64_t(1) << best;
+            if (bestbit >= 0 && !((S >> bestbit) & 1u)) {
+                S |= 1u << bestbit;
+                ++nS;
+            }
+            mem.push_back({best, bestbit});
+        }
+        if (mem.empty()) return fail(SVX_E_FORMAT, "internal: register grouping made no progress");
+        std::vector<int> rb;
+        for (int b = 0; b < K; ++b)
+            if ((S >> b) & 1u) rb.push_back(b);
+        for (int b = K - 1; b >= 0 && (int)rb.size() < R; --b)
+            if (!((S >> b) & 1u)) rb.push_back(b);
+        GroupDev gd{};
+        gd.nops = (int)mem.size();
+        gd.op0 = (int)ops.size();
+        uint32_t regmask = 0;
+        for (int q = 0; q < R; ++q) {
+            gd.b[q] = gd.bs[q] = rb[q];
+            regmask |= 1u << rb[q];
+        }
+        std::sort(gd.bs, gd.bs + R);
+        for (const auto &mb : mem) {
+            const int i = mb.first;
+            OpDev<T> od{};
+            if (dense_of[i] >= 0) {
+                od.kind = (M_TABLE - M_FULL) * 8;
+                od.aux = (int32_t)put_table_regs(dus[dense_of[i]], rb);
+                ops.push_back(od);
+                continue;
+            }
+            const OpDev<T> &src = pops[i];
+            const int t = mb.second;
+            uint32_t cmask = (uint32_t)src.cmask;
+            if (t != src.t) cmask = (cmask | (1u << src.t)) & ~(1u << t);   // controlled phase on another of its bits
+            od.t = t;
+            od.cmask = (int32_t)cmask;
+            od.tr = (int)(std::find(rb.begin(), rb.end(), t) - rb.begin());
+            const bool unc = src.kind <= K_XSWAP;
+            uint32_t cmr = 0;
+            for (int q = 0; q < R; ++q)
+                if ((cmask >> rb[q]) & 1u) cmr |= 1u << q;
+            od.cmr = unc ? 0u : cmr;
+            od.cmt = unc ? 0u : (cmask & ~regmask);
+            const T *m0 = src.m0, *m1 = src.m1;
+            switch (src.kind) {
+            case K_FULL: {
+                const bool realm = m0[1] == 0 && m0[3] == 0 && m0[5] == 0 && m0[7] == 0;
+                if (realm) {
+                    od.kind = M_REAL;
+                    od.m0[0] = m0[0];
+                    od.m0[1] = m0[2];
+                    od.m0[2] = m0[4];
+                    od.m0[3] = m0[6];
+                } else {
+                    // consecutive general ops alternate between the matrix cores and the vector
+                    // units, so the waves of a SIMD keep both pipes busy
+                    od.kind = (mfma && (n_full++ & 1) == 0) ? M_FULLM : M_FULL;
+                    T *w = od.m0;   // 16 floats over m0, m1
+                    for (int r = 0; r < 2; ++r) {
+                        const T ar = m0[4 * r], ai = m0[4 * r + 1], br = m0[4 * r + 2], bi = m0[4 * r + 3];
+                        T *re = w + 8 * r, *im = w + 8 * r + 4;
+                        re[0] = ar; re[1] = -ai; re[2] = br; re[3] = -bi;
+                        im[0] = ai; im[1] = ar; im[2] = bi; im[3] = br;
+                    }
+                }
+                break;
+            }
+            case K_DIAG:
+                od.kind = M_DIAG;
+                od.m0[0] = m0[0];
+                od.m0[1] = m0[1];
+                od.m0[2] = m0[6];
+                od.m0[3] = m0[7];
+                break;
+            case K_XSWAP: od.kind = M_X; break;
+            case K_CXSWAP:
+            case K_CXSWAP1:
+            case K_CPHASE:
+            case K_CPHASE1: {
+                // one register control and none outside: the touched pairs are fixed (CB in the cmr
+                // slot); controls only outside: one lane mask; otherwise the general selection
+                const bool phase = src.kind == K_CPHASE || src.kind == K_CPHASE1;
+                if (od.cmt == 0 && __builtin_popcount(od.cmr) == 1) {
+                    od.kind = phase ? M_CPHASE1 : M_CX1;
+                    od.cmr = (uint32_t)__builtin_ctz(od.cmr);
+                } else if (od.cmr == 0) {
+                    od.kind = phase ? M_CPHASET : M_CXT;
+                } else {
+                    od.kind = phase ? M_CPHASE : M_CX;
+                }
+                if (phase) {
+                    od.m0[0] = m1[6];
+                    od.m0[1] = m1[7];
+                }
+                break;
+            }
+            default:
+                od.kind = M_CGEN;
+                for (int e = 0; e < 8; ++e) {
+                    od.m0[e] = m0[e];
+                    od.m1[e] = m1[e];
+                }
+                break;
+            }
+            auto rows = [](const T *mm) {
+                const double a = std::hypot((double)mm[0], (double)mm[1]) + std::hypot((double)mm[2], (double)mm[3]);
+                const double b = std::hypot((double)mm[4], (double)mm[5]) + std::hypot((double)mm[6], (double)mm[7]);
+                return std::max(a, b);
+            };
+            log2_growth += std::log2(std::max(1.0, std::max(rows(m0), rows(m1))));
+            if (od.kind == M_FULLM) ++*n_mfma;
+            od.kind = (od.kind - M_FULL) * 8 + od.tr;   // the kernel dispatches on kind and target at once
+            ops.push_back(od);
+        }
+        groups.push_back(gd);
+    }
+    return 0;
+}
+
 template <typename T>
 int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Prepared *pr,
                    std::vector<unsigned char> *blob) {
@@ -871,7 +1056,11 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
         else mode = 1;   // light and heavy passes alike: the single-buffer kernel
         (void)pick_single;
         const int K = mode == 0 ? nbits : (mode == 1 ? Kv1 : k);
-        const int R = mode == 2 ? 4 : kR;
+        // complex64 passes without dense units (diagonal tables allowed) run on the FP32 MUX2 kernel
+        bool has_units = false;
+        for (size_t i = 0; i < dense_of.size(); ++i) has_units = has_units || (dense_of[i] >= 0 && !dus[dense_of[i]].diag);
+        const bool use_mux = mode == 1 && dtype == SVX_C64 && !has_units;
+        const int R = mode == 2 ? 4 : (use_mux ? mux_register_bits(K) : kR);
         PassDev pd{};
         pd.k = k;
         pd.m = rp.m;
@@ -921,7 +1110,8 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
                 return fail(SVX_E_CAPACITY, "2^" + std::to_string(nbits - K) + " workgroup tiles of 2^" + std::to_string(K) +
                                                 " amplitudes exceed the launch limit of 2^32 work-items");
             li.grid = (uint32_t)li.ntiles;
-            li.block = kNT;
+            li.block = use_mux ? (1u << (K - R)) : kNT;
+            li.mux = use_mux;
             li.lds = pass_lds_bytes(K, rp.m, k, (int)sizeof(T) * 2, false, nbits);
         }
         launches.push_back(li);
@@ -995,6 +1185,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             return off;
         };
         auto is_table = [&](size_t x) { return dense_of[x] >= 0 && dus[dense_of[x]].diag; };
+        if (use_mux) {
+            const int rc = mux_groups<T>(pops, dense_of, dus, K, R, opt.mux == SVX_MUX_AUTO, groups, ops, put_table_regs,
+                                         log2_growth, &launches.back().stats[5]);
+            if (rc) return rc;
+            i = pops.size();
+        }
         while (i < pops.size()) {
             if (is_table(i)) {
                 // a table with no MUX2 op before the next dense unit (or the end): its own group, the
@@ -1187,7 +1383,8 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             const GroupDev &G = groups[g];
             if (G.type == 0) {
                 int nt = 0;
-                for (int o = 0; o < G.nops; ++o) nt += ops[G.op0 + o].kind == K_TABLE;
+                for (int o = 0; o < G.nops; ++o)
+                    nt += lr.mux ? ops[G.op0 + o].kind == (M_TABLE - M_FULL) * 8 : ops[G.op0 + o].kind == K_TABLE;
                 lr.stats[0] += 1;
                 lr.stats[1] += G.nops - nt;
                 lr.stats[6] += nt;
@@ -1377,6 +1574,9 @@ hipError_t launch_one(void *state, const Prepared *pr, const LaunchInfo &li, hip
     if (li.dma)
         return launch_pass_dma(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
                                li.ntiles, li.grid, li.lds, s);
+    if (li.mux)
+        return launch_mux(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd, li.grid,
+                          li.lds, s);
     if (pr->dtype == SVX_C64)
         return launch_pass_c64(li.K, li.small, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
                                wd, li.grid, li.lds, s);
@@ -1603,7 +1803,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         uint64_t tiles = 0;
         size_t lds = 0;
     };
-    std::map<int, Cls> cls;   // key: K * 4 + dense + 2 d4
+    std::map<int, Cls> cls;   // key: K * 8 + dense + 2 d4 + 4 mux
     std::vector<Prepared *> used;
     {
         std::set<void *> seen;
@@ -1633,7 +1833,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
             it.gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
             it.od = base + pr->op_off;
             it.wd = li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr;
-            Cls &c = cls[li.K * 4 + (li.dense ? 1 : 0) + (li.d4 ? 2 : 0)];
+            Cls &c = cls[li.K * 8 + (li.dense ? 1 : 0) + (li.d4 ? 2 : 0) + (li.mux ? 4 : 0)];
             it.tile0 = c.tiles;
             c.tiles += li.ntiles;
             c.lds = std::max(c.lds, li.lds);
@@ -1649,7 +1849,9 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
     }
     for (auto &kv : cls) {
         Cls &c = kv.second;
-        const uint64_t threads = (kv.first & 2) ? kNTD4 : kNT;
+        const int bK = kv.first / 8;
+        const bool bmux = (kv.first & 4) != 0;
+        const uint64_t threads = bmux ? (uint64_t(1) << (bK - mux_register_bits(bK))) : ((kv.first & 2) ? kNTD4 : kNT);
         if (c.tiles * threads > 0xffffffffull)
             return fail(SVX_E_CAPACITY, "batch launch beyond the launch limit of 2^32 work-items");
         const int slot = ds->batch_next;
@@ -1659,8 +1861,9 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         BatchItem *dv = ds->batch_dev + (size_t)slot * DeviceState::kBatchCap;
         std::memcpy(hs, c.items.data(), c.items.size() * sizeof(BatchItem));
         HIPCHK(hipMemcpyAsync(dv, hs, c.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, s));
-        const hipError_t e = launch_pass_batch(kv.first / 4, kv.first & 1, (kv.first & 2) != 0, dv, (int)c.items.size(),
-                                               (uint32_t)c.tiles, c.lds, s);
+        const hipError_t e = bmux ? launch_mux_batch(bK, dv, (int)c.items.size(), (uint32_t)c.tiles, c.lds, s)
+                                  : launch_pass_batch(bK, kv.first & 1, (kv.first & 2) != 0, dv, (int)c.items.size(),
+                                                      (uint32_t)c.tiles, c.lds, s);
         if (e != hipSuccess) return hipfail(e, "batched pass launch");
         HIPCHK(hipEventRecord(ds->batch_ev[slot], s));
         ds->batch_used[slot] = true;
@@ -1694,6 +1897,7 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
     else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dense ? ",dense>" : ">");
     else if (li.d4) k = "k_pass_d4<" + std::to_string(li.K) + ">";
+    else if (li.mux) k = "k_mux<" + std::to_string(li.K) + "," + std::to_string(mux_register_bits(li.K)) + ">";
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");
     std::snprintf(name, cap, "%s", k.c_str());
     return 0;

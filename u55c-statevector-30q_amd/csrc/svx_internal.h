@@ -30,6 +30,21 @@ enum OpKind : int32_t {
     K_DENSE = 10,  // small-state kernel only: dense op, cmask = index of its GroupDev (type 2)
     K_TABLE = 11,  // complex64 pass kernel: diagonal table, 2^k complex64 phases at float offset aux
                    // of the dense operand array, indexed by the QSV2 tile index (no register bits)
+    // op kinds of the FP32 MUX2 kernel (k_mux); the 16 floats m0, m1 hold the coefficients, and the
+    // record's kind field holds (kind - M_FULL) * 8 + the target register bit
+    M_FULL = 32,   // uncontrolled 2x2: w[4 i + j] = real 4x4 form, rows (y0r y0i y1r y1i), cols (x0r x0i x1r x1i)
+    M_REAL = 33,   // uncontrolled real 2x2: w = (m00, m01, m10, m11)
+    M_DIAG = 34,   // uncontrolled diagonal: w = (d0r, d0i, d1r, d1i)
+    M_X = 35,      // uncontrolled exact X (a register exchange)
+    M_CX = 36,     // controlled exact X (M0 = I): selected pairs exchange
+    M_CPHASE = 37, // controlled diag(1, e) (M0 = I): w = (er, ei)
+    M_CGEN = 38,   // any other controlled op: selected pairs get M1, the others M0 (complex64 m0, m1)
+    M_TABLE = 39,  // diagonal table (aux: float offset of the slot-major table)
+    M_FULLM = 40,  // M_FULL with half of the pairs on the FP32 matrix cores (v_mfma_f32_4x4x1_16b_f32)
+    M_CX1 = 41,    // controlled exact X, one control among the register bits (aux: its register bit), none outside
+    M_CXT = 42,    // controlled exact X, controls only outside the register bits (one lane mask for all pairs)
+    M_CPHASE1 = 43,  // controlled diag(1, e), one control among the register bits (aux), none outside
+    M_CPHASET = 44,  // controlled diag(1, e), controls only outside the register bits
 };
 
 struct PassDev {
@@ -102,6 +117,16 @@ hipError_t launch_pass_c64(int K, bool small, void *st, const PassDev *pd, const
 hipError_t launch_pass_c128(int K, bool small, void *st, const PassDev *pd, const GroupDev *gd,
                             const OpDev<double> *od, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t set_kernel_attributes();
+// FP32 MUX2 kernel (complex64 passes of MUX2 ops and diagonal tables): 2^(K - R) threads, R register bits
+#ifndef SVX_MUX_R13
+#define SVX_MUX_R13 4
+#endif
+constexpr int kMuxR13 = SVX_MUX_R13;   // register bits at K = 13 (4: 512 threads, 5: 256 threads)
+int mux_register_bits(int K);
+hipError_t set_mux_attributes();
+hipError_t launch_mux(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+                      const float *wd, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_mux_batch(int K, const BatchItem *items, int n_items, uint32_t grid, size_t lds, hipStream_t s);
 // double-buffered complex64 pass kernel (one QSV2 tile of 2^K per LDS buffer, K in {12, 13});
 // register groups of 4 tile bits
 size_t pass_dma_lds_bytes(int K, int m);
