@@ -157,6 +157,8 @@ def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
             libsvx.run_prepared(state, h)
         e1.record(stream)
         torch.cuda.synchronize()
+        for h in hs:
+            libsvx.release_prepared(h)
         return e0.elapsed_time(e1), st, fmap, prog
 
     plain_ms, st_p, fmap_p, prog_p = timed(13, "plain")
@@ -172,6 +174,8 @@ def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
            "fused_split16_optin_ms": None if split_ms is None else round(split_ms, 1),
            "tuned": {"k": rec["k"], "form": rec["form"], "tune_s": round(t_tune, 1),
                      "calibration_ms": rec["table"]}}
+    norm, _ = libsvx.reduce_f64(state)
+    out["fused_norm_f64"] = norm
     if judge:
         assert fmap_p == fmap_f == list(range(n)), "the layer recipe compiles without relabelling"
         s128 = torch.empty(1 << n, dtype=torch.complex128, device="cuda:0")
@@ -179,8 +183,6 @@ def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
         for _, w in prog_p:
             libsvx.run_segment(s128, w)
         out["fused_vs_c128_plain"] = libsvx.fidelity_f64(s128, state)
-        norm, _ = libsvx.reduce_f64(state)
-        out["fused_norm"] = norm
         del s128
     del state
     torch.cuda.empty_cache()
@@ -248,6 +250,8 @@ def main():
     ap.add_argument("--no-fused", action="store_true", help="skip the 32/34-qubit fused-vs-plain section")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix section")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served-mix section (e.g. --serve-seconds 600 for the 10-minute stream)")
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--ladder-probe", action="store_true",
                     help="for rocprofv3 kernel traces: the 30-qubit controlled-phase ladder's complex64 passes "
@@ -271,6 +275,9 @@ def main():
     from u55c_statevector_30q_amd import circuit, compiler, libsvx, runtime, serving
 
     caps = libsvx.init(0)
+    if a.serve_only:
+        print(json.dumps({"served_mix": served_section(serving, torch, seconds=a.serve_seconds)}))
+        return
     n = 30
     qc = circuit.workload_gpu30()
     if a.ladder_probe:
