@@ -192,6 +192,16 @@ int svx_product_state(void *state, int32_t n, int32_t dtype, const double *amps,
 int svx_reduce_f64(const void *state, int32_t n, int32_t dtype, const int32_t *fmap, int32_t n_low,
                    double *out_norm, double *out_marginals, void *stream);
 
+/* svx_reduce_f64 without allocation, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments, kernels, reduction order and results as svx_reduce_f64; the FP64
+ * partials, the bin configuration and the bins live in a library-owned static device array of
+ * 4 MiB + 6 KiB per device (held until the process ends), so n_low is limited to 8 (SVX_E_CAPACITY
+ * beyond).  Every call on one device shares that array: calls that overlap on one GPU race.
+ * Synchronises `stream`. */
+int svx_reduce_f64_noalloc(const void *state, int32_t n, int32_t dtype, const int32_t *fmap, int32_t n_low,
+                           double *out_norm, double *out_marginals, void *stream);
+
 /* Judge metrics in FP64 (tests/functional_suite.py:90-94, bench/fpga/sv_maxwidth_ref.py:15-25):
  * out[0] = F = |<a|b>|^2 / (<a|a><b|b>), out[1] = 1-F = d^2 - d^4/4 after global-phase alignment
  * of the normalised states (no cancellation), out[2] = max |b/|b| - ph a/|a||, out[3] = d.

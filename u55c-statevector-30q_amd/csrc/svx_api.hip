@@ -159,9 +159,17 @@ class Arena {
     size_t held_ = 0;
 };
 
+// Scratch of svx_reduce_f64_noalloc: one copy per device, kept until the process ends.  Host code
+// uses only the address looked up per device (DeviceState::scratch), never this name.
+constexpr int kScratchMaxLow = 8;                                        // at most 2^8 bins
+constexpr size_t kScratchPartBytes = size_t(2048) << kScratchMaxLow << 3;  // 2048 workgroups x 256 bins x FP64
+constexpr size_t kScratchBytes = kScratchPartBytes + 4096 + (size_t(8) << kScratchMaxLow);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct DeviceState {
     bool ready = false;
     svx_caps caps{};
+    unsigned char *scratch = nullptr;   // this device's address of the scratch array
     // library scratch for reductions and product-state tables
     double *part = nullptr;
     size_t part_bytes = 0;
@@ -198,6 +206,10 @@ int ensure_device(int dev, DeviceState **out) {
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             return fail(SVX_E_NODEV, std::string("libsvx is built for gfx950, device is ") + prop.gcnArchName);
         HIPCHK(set_kernel_attributes());
+        // this device's copy of the scratch array (the allocation-free reduction works in it)
+        void *sp = nullptr;
+        HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        ds->scratch = static_cast<unsigned char *>(sp);
         ds->caps.device = dev;
         ds->caps.cu_count = prop.multiProcessorCount;
         ds->caps.lds_per_block = (int64_t)prop.sharedMemPerBlock;
@@ -1677,6 +1689,46 @@ int build_perm(int n, const int32_t *perm, PermDev *pd, uint32_t *grid) {
     return 0;
 }
 
+// Bin configuration of the FP64 marginal reduction (k_marg) and its grid.
+int marg_cfg(int n, const int32_t *fmap, int n_low, MargCfg *cfg, uint32_t *grid) {
+    std::memset(cfg, 0, sizeof(*cfg));
+    const int c = std::min(n, 13);
+    cfg->n = n;
+    cfg->c = c;
+    cfg->low8 = std::min(c, 8);
+    cfg->nlow = n_low;
+    int used[64];
+    for (int b = 0; b < n; ++b) used[b] = -1;
+    for (int i = 0; i < n_low; ++i) {
+        if (fmap[i] < 0 || fmap[i] >= n || used[fmap[i]] != -1) return fail(SVX_E_ARG, "fmap entries must be distinct bits < n");
+        used[fmap[i]] = i;
+    }
+    for (int b = 0; b < cfg->low8; ++b) {
+        if (used[b] >= 0) {
+            cfg->sa[cfg->nsa] = b;
+            cfg->sa_x[cfg->nsa++] = used[b];
+        } else {
+            cfg->ra[cfg->nra++] = b;
+        }
+    }
+    for (int b = cfg->low8; b < c; ++b) {
+        if (used[b] >= 0) {
+            cfg->sb[cfg->nsb] = b;
+            cfg->sb_x[cfg->nsb++] = used[b];
+        } else {
+            cfg->rb[cfg->nrb++] = b;
+        }
+    }
+    for (int b = c; b < n; ++b)
+        if (used[b] >= 0) {
+            cfg->shi[cfg->nshi] = b - c;
+            cfg->shi_x[cfg->nshi++] = used[b];
+        }
+    const uint64_t nchunk = uint64_t(1) << (n - c);
+    *grid = (uint32_t)std::min<uint64_t>(nchunk, 2048);
+    return 0;
+}
+
 }  // namespace
 
 // ============================================================================================ C ABI
@@ -2099,41 +2151,10 @@ int svx_reduce_f64(const void *state, int32_t n, int32_t dtype, const int32_t *f
     int rc = get_ds(&ds);
     if (rc) return rc;
     MargCfg cfg;
-    std::memset(&cfg, 0, sizeof(cfg));
-    const int c = std::min(n, 13);
-    cfg.n = n;
-    cfg.c = c;
-    cfg.low8 = std::min(c, 8);
-    cfg.nlow = n_low;
-    std::vector<int> used(n, -1);
-    for (int i = 0; i < n_low; ++i) {
-        if (fmap[i] < 0 || fmap[i] >= n || used[fmap[i]] != -1) return fail(SVX_E_ARG, "fmap entries must be distinct bits < n");
-        used[fmap[i]] = i;
-    }
-    for (int b = 0; b < cfg.low8; ++b) {
-        if (used[b] >= 0) {
-            cfg.sa[cfg.nsa] = b;
-            cfg.sa_x[cfg.nsa++] = used[b];
-        } else {
-            cfg.ra[cfg.nra++] = b;
-        }
-    }
-    for (int b = cfg.low8; b < c; ++b) {
-        if (used[b] >= 0) {
-            cfg.sb[cfg.nsb] = b;
-            cfg.sb_x[cfg.nsb++] = used[b];
-        } else {
-            cfg.rb[cfg.nrb++] = b;
-        }
-    }
-    for (int b = c; b < n; ++b)
-        if (used[b] >= 0) {
-            cfg.shi[cfg.nshi] = b - c;
-            cfg.shi_x[cfg.nshi++] = used[b];
-        }
+    uint32_t grid = 0;
+    rc = marg_cfg(n, fmap, n_low, &cfg, &grid);
+    if (rc) return rc;
     const int nbin = 1 << n_low;
-    const uint64_t nchunk = uint64_t(1) << (n - c);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(nchunk, 2048);
     std::lock_guard<std::mutex> lk(ds->mu);
     rc = ensure_scratch(ds, std::max<size_t>((size_t)grid * 256 * sizeof(double), 1 << 20), 1 << 16);
     if (rc) return rc;
@@ -2144,13 +2165,54 @@ int svx_reduce_f64(const void *state, int32_t n, int32_t dtype, const int32_t *f
     HIPCHK(hipMemcpy(cfg_dev, &cfg, sizeof(cfg), hipMemcpyHostToDevice));
     hipError_t e = launch_marginals(dtype, state, cfg_dev, nbin, ds->part, grid, out_dev, s);
     if (e != hipSuccess) return hipfail(e, "marginal launch");
-    std::vector<double> out(nbin);
-    HIPCHK(hipMemcpyAsync(out.data(), out_dev, nbin * sizeof(double), hipMemcpyDeviceToHost, s));
+    double out[256];
+    HIPCHK(hipMemcpyAsync(out, out_dev, nbin * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     double norm = 0;
     for (int x = 0; x < nbin; ++x) norm += out[x];
     if (out_norm) *out_norm = norm;
-    if (n_low > 0) std::memcpy(out_marginals, out.data(), nbin * sizeof(double));
+    if (n_low > 0) std::memcpy(out_marginals, out, nbin * sizeof(double));
+    return 0;
+}
+
+// svx_reduce_f64 without allocation: the FP64 partials, the bin configuration and the folded bins
+// live in this device's copy of the file-scope `scratch` array (its address looked up at device
+// set-up).  Same kernels, grid and reduction order as svx_reduce_f64, so the results are identical.
+int svx_reduce_f64_noalloc(const void *state, int32_t n, int32_t dtype, const int32_t *fmap, int32_t n_low,
+                           double *out_norm, double *out_marginals, void *stream) {
+    if (!state || n < 1 || n > 40 || n_low < 0 || n_low > n) return fail(SVX_E_ARG, "bad reduce arguments");
+    if (n_low > kScratchMaxLow)
+        return fail(SVX_E_CAPACITY, "svx_reduce_f64_noalloc: 2^" + std::to_string(n_low) + " bins exceed its " +
+                                        std::to_string(kScratchBytes) + "-byte scratch array (at most 2^" +
+                                        std::to_string(kScratchMaxLow) + " bins of 2048 workgroups)");
+    if (n_low > 0 && (!fmap || !out_marginals)) return fail(SVX_E_ARG, "fmap and out_marginals required when n_low > 0");
+    if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "bad dtype");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    if (!ds->scratch) return fail(SVX_E_HIP, "scratch array address unavailable on this device");
+    MargCfg cfg;
+    uint32_t grid = 0;
+    rc = marg_cfg(n, fmap, n_low, &cfg, &grid);
+    if (rc) return rc;
+    const int nbin = 1 << n_low;
+    if ((size_t)grid * nbin * sizeof(double) > kScratchPartBytes)
+        return fail(SVX_E_CAPACITY, "svx_reduce_f64_noalloc: partials exceed the scratch array");
+    // layout of the scratch array: [partials: kScratchPartBytes][MargCfg: 4 KiB][bins: 256 doubles]
+    double *part = reinterpret_cast<double *>(ds->scratch);
+    MargCfg *cfg_dev = reinterpret_cast<MargCfg *>(ds->scratch + kScratchPartBytes);
+    double *out_dev = reinterpret_cast<double *>(ds->scratch + kScratchPartBytes + 4096);
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipMemcpyAsync(cfg_dev, &cfg, sizeof(cfg), hipMemcpyHostToDevice, s));
+    hipError_t e = launch_marginals(dtype, state, cfg_dev, nbin, part, grid, out_dev, s);
+    if (e != hipSuccess) return hipfail(e, "marginal launch");
+    double out[1 << kScratchMaxLow];
+    HIPCHK(hipMemcpyAsync(out, out_dev, nbin * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double norm = 0;
+    for (int x = 0; x < nbin; ++x) norm += out[x];
+    if (out_norm) *out_norm = norm;
+    if (n_low > 0) std::memcpy(out_marginals, out, nbin * sizeof(double));
     return 0;
 }
 

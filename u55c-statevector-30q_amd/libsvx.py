@@ -24,7 +24,7 @@ SYMBOLS = ["svx_init", "svx_run_segment", "svx_run_segment_ex", "svx_engine", "s
            "svx_prepare", "svx_prepare_ex", "svx_release_prepared", "svx_cache_limit", "svx_prepared_shape",
            "svx_run_prepared", "svx_tune_prepared", "svx_prepared_info", "svx_prepared_kernel",
            "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state",
-           "svx_product_state", "svx_reduce_f64", "svx_fidelity_f64", "svx_convert", "svx_sync",
+           "svx_product_state", "svx_reduce_f64", "svx_reduce_f64_noalloc", "svx_fidelity_f64", "svx_convert", "svx_sync",
            "svx_last_error", "svx_version"]
 
 FUSE = {"auto": 0, "on": 0, "off": 1, "dense4": 2}
@@ -123,6 +123,7 @@ _SIGS = {
     "svx_zero_state": ([_VP, _I32, _I32, _VP], _I32),
     "svx_product_state": ([_VP, _I32, _I32, _PD, _VP], _I32),
     "svx_reduce_f64": ([_VP, _I32, _I32, _PI32, _I32, _PD, _PD, _VP], _I32),
+    "svx_reduce_f64_noalloc": ([_VP, _I32, _I32, _PI32, _I32, _PD, _PD, _VP], _I32),
     "svx_fidelity_f64": ([_VP, _I32, _VP, _I32, _I32, _PD, _VP], _I32),
     "svx_convert": ([_VP, _I32, _VP, _I32, _I64, _VP], _I32),
     "svx_sync": ([_VP], _I32),
@@ -370,6 +371,20 @@ def reduce_f64(state, fmap=(), stream=None):
     marg = np.zeros(1 << nl, dtype=np.float64)
     _check(lib.svx_reduce_f64(_dev(state), n, dtype_code(state), fp, nl, ctypes.byref(norm),
                               marg.ctypes.data_as(_PD), _stream(stream)))
+    return norm.value, marg
+
+
+def reduce_f64_noalloc(state, fmap=(), stream=None):
+    """svx_reduce_f64_noalloc: reduce_f64 working in the library's static scratch array (no device
+    allocation; at most 8 marginal qubits; overlapping calls on one GPU race)."""
+    lib = load()
+    n = _nbits(state)
+    f, fp = _i32(list(fmap) if len(fmap) else [0])
+    nl = len(fmap)
+    norm = ctypes.c_double(0)
+    marg = np.zeros(1 << nl, dtype=np.float64)
+    _check(lib.svx_reduce_f64_noalloc(_dev(state), n, dtype_code(state), fp, nl, ctypes.byref(norm),
+                                      marg.ctypes.data_as(_PD), _stream(stream)))
     return norm.value, marg
 
 
