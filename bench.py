@@ -58,11 +58,137 @@ def cpu_baseline(max_seconds=30.0):
     med = float(np.median(timed))
     sweeps = st.n_passes + st.n_remaps
     gbs = sweeps * 2 * (1 << 20) * 8 / med / 1e9
+    # the complex128 variant of the same records (the judge's arithmetic), bounded the same way
+    t128 = []
+    t_start = time.perf_counter()
+    for rep in range(3):
+        a = time.perf_counter()
+        model.run_program(prog, 20, 8, fmap, ctype=np.complex128)
+        t128.append(time.perf_counter() - a)
+        if time.perf_counter() - t_start > max_seconds / 2 and rep >= 1:
+            break
+    med128 = float(np.median(t128[1:] if len(t128) > 1 else t128))
     return dict(value=round(gbs, 4), unit="GB/s", cores=1, kind="port",
                 sample=(f"golden model (numpy c64) on the 20-qubit workload (depth 20, seed 0): "
                         f"{st.n_ops} ops, {st.n_passes} passes + {st.n_remaps} remaps at P=8, L=17, k=12; "
                         f"median of {len(timed)} runs after 1 warm-up = {med:.3f} s; compile {t_compile:.3f} s; "
-                        f"pinned to core {core} of {os.cpu_count()}"))
+                        f"pinned to core {core} of {os.cpu_count()}"),
+                c64_s=round(med, 4),
+                c128={"value": round(sweeps * 2 * (1 << 20) * 16 / med128 / 1e9, 4), "unit": "GB/s",
+                      "s": round(med128, 4), "runs": len(t128)})
+
+
+def gpu_same_records(libsvx, circuit, compiler, torch, np, reps=5):
+    """The cpu_baseline's records (20 qubits, reference stream: P = 8 slices of 2^17, k = 12, MAXH 4;
+    passes and remaps) on the device: svx_run_segment with P slices and one permutation sweep per
+    remap, complex64 and complex128, median of `reps` runs after one warm-up (HIP events)."""
+    qc = circuit.workload_cpu20()
+    prog, fmap, st = compiler.compile_circuit(qc, 17, 12, max_high=4)
+    out = {}
+    stream = torch.cuda.current_stream()
+    for name, dt in (("c64", torch.complex64), ("c128", torch.complex128)):
+        a = torch.empty(1 << 20, dtype=dt, device="cuda:0")
+        b = torch.empty_like(a)
+        ms = []
+        for rep in range(reps + 1):
+            libsvx.zero_state(a)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            cur, oth = a, b
+            for kind, payload in prog:
+                if kind == "segment":
+                    libsvx.run_segment(cur, payload, n_slices=8)
+                else:
+                    libsvx.permute(oth, cur, payload)
+                    cur, oth = oth, cur
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if rep:
+                ms.append(e0.elapsed_time(e1))
+        med = float(np.median(ms))
+        sweeps = st.n_passes + st.n_remaps
+        out[name] = {"ms": round(med, 3),
+                     "gbs": round(sweeps * 2 * (1 << 20) * (8 if name == "c64" else 16) / (med * 1e-3) / 1e9, 1)}
+    out["records"] = f"{st.n_passes} passes + {st.n_remaps} remaps, P=8, L=17, k=12 (the cpu_baseline's stream)"
+    return out
+
+
+U55C_FLAGSHIP = {"ghz": (141, 1.3), "qft": (102, 15.8), "rqc": (67, 13.3), "vqe": (80, 4.7)}
+
+
+def families_section(runtime, circuit, torch, np, n=30, reps=3):
+    """The reference's 30-qubit flagship families (GHZ, QFT, RQC depth 10, VQE 2 layers) through
+    SVRuntime.run on the default FP32 path, timed like the reference's records (T_kernel = time of the
+    segment launches, T_e2e = the run call; median of `reps` after one warm-up run), and the sweep
+    bandwidth by the reference's definition passes x 2^30 x 16 B / T_kernel
+    (analysis/make_tables.py:320-321).  The U55C's medians (paper/generated/macros.tex:33-57) are
+    context: a different device, and the reference's plan has more passes and remaps."""
+    rt = runtime.SVRuntime()
+    out = {}
+    for fam in ("ghz", "qft", "rqc", "vqe"):
+        qc = circuit.make(fam, n)
+        rt.run(qc, shots=1024, return_state=False, seed=1)
+        runs = [rt.run(qc, shots=1024, return_state=False, seed=1) for _ in range(reps)]
+        tk = float(np.median([r["timing"]["T_kernel"] for r in runs]))
+        te = float(np.median([r["timing"]["T_e2e"] for r in runs]))
+        stt = runs[0]["stats"]
+        passes = stt["n_passes"]
+        u_bw, u_tk = U55C_FLAGSHIP[fam]
+        out[fam] = {"ops": stt["n_ops"], "passes": passes, "remaps": stt["n_remaps"],
+                    "T_kernel_s": round(tk, 4), "T_e2e_s": round(te, 4),
+                    "sweep_gbs": round(passes * (1 << 30) * 16 / tk / 1e9, 1),
+                    "u55c_sweep_gbs": u_bw, "u55c_T_kernel_s": u_tk}
+    del rt
+    torch.cuda.empty_cache()
+    return out
+
+
+def latency_section(runtime, circuit, np, runs=2000, warmup=20):
+    """Serial latency (bench/latency_jitter.py:24-58): RQC-16 (depth 10) through SVRuntime.run with
+    1024 shots and no state return, `runs` timed calls after `warmup`; p50 / p99 / max of the
+    end-to-end time and of T_kernel, and the spread (p99 - p50) / p50.  The U55C's record is
+    p50 / p99 = 32.4 / 36.9 ms (paper/generated/macros.tex:114-115)."""
+    rt = runtime.SVRuntime()
+    qc = circuit.make("rqc", 16)
+    e2e, ker = [], []
+    for i in range(runs + warmup):
+        t0 = time.perf_counter()
+        r = rt.run(qc, shots=1024, return_state=False)
+        t = time.perf_counter() - t0
+        if i >= warmup:
+            e2e.append(t * 1e3)
+            ker.append(r["timing"]["T_kernel"] * 1e3)
+    x, k = np.array(e2e), np.array(ker)
+    p50, p99 = float(np.percentile(x, 50)), float(np.percentile(x, 99))
+    return {"workload": f"rqc_16_d10 through SVRuntime.run, 1024 shots, {runs} runs after {warmup}",
+            "e2e_p50_ms": round(p50, 3), "e2e_p99_ms": round(p99, 3), "e2e_max_ms": round(float(x.max()), 3),
+            "spread_pct": round(100 * (p99 - p50) / p50, 1),
+            "kernel_p50_ms": round(float(np.percentile(k, 50)), 3),
+            "kernel_p99_ms": round(float(np.percentile(k, 99)), 3),
+            "u55c_p50_p99_ms": [32.4, 36.9]}
+
+
+def hbm_roof_section(libsvx, torch, reps=10):
+    """Measured HBM roof: svx_convert of an 8 GiB complex64 state into another (a same-type copy:
+    the library's streaming copy kernel, 16-byte non-temporal accesses), mean of `reps` launches."""
+    a = torch.zeros(1 << 30, dtype=torch.complex64, device="cuda:0")
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        libsvx.convert(b, a)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        libsvx.convert(b, a)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    del a, b
+    torch.cuda.empty_cache()
+    m = sum(ms) / len(ms)
+    return {"copy_bytes": 2 * (1 << 33), "mean_ms": round(m, 4), "best_ms": round(min(ms), 4),
+            "gbs_mean": round(2 * (1 << 33) / (m * 1e-3) / 1e9, 1),
+            "gbs_best": round(2 * (1 << 33) / (min(ms) * 1e-3) / 1e9, 1)}
 
 
 def cp_ladder_section(libsvx, circuit, compiler, torch, np, n=30, seed=300):
@@ -112,6 +238,15 @@ def cp_ladder_section(libsvx, circuit, compiler, torch, np, n=30, seed=300):
     back = torch.empty_like(s64)
     libsvx.permute(back, t64, rev)
     exact = bool(torch.equal(back, s64))
+    # the same reversal in place (no second state): a few tile-local sweeps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    libsvx.permute(back, back, rev)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    inplace_ms = e0.elapsed_time(e1)
+    inplace_exact = bool(torch.equal(back, t64))
+    sweeps = len(libsvx.permute_plan(rev))
     t128 = torch.empty_like(s128)
     libsvx.permute(t128, s128, rev)
     final = libsvx.fidelity_f64(t128, t64)
@@ -124,11 +259,14 @@ def cp_ladder_section(libsvx, circuit, compiler, torch, np, n=30, seed=300):
             "c64_pass_gbs": round(len(single) * nb / (times["c64"] * 1e-3) / 1e9, 1),
             "bitrev_ms": round(perm_ms, 3), "bitrev_gbs": round(nb / (perm_ms * 1e-3) / 1e9, 1),
             "bitrev_roundtrip_bit_exact": exact,
+            "bitrev_in_place": {"ms": round(inplace_ms, 3), "sweeps": sweeps,
+                                "gbs_per_sweep": round(sweeps * nb / (inplace_ms * 1e-3) / 1e9, 1),
+                                "equals_out_of_place": inplace_exact},
             "final_vs_c128": final,
             "error_growth_pass_1mF_maxabs": [[g[0], float(f"{g[1]:.3e}"), float(f"{g[2]:.3e}")] for g in growth]}
 
 
-def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
+def fused_section(libsvx, circuit, runtime, torch, np, n, arch, judge=True):
     """Fused vs plain on the n-qubit fused-form workload (layer recipe, depth 40, seed 2),
     complex64: the plain MUX2 records with every op on the vector units against the autotuned
     form (tile width x {plain, library fusion, fused 4-/5-qubit records}, timed on a depth-4
@@ -174,8 +312,27 @@ def fused_section(libsvx, circuit, runtime, torch, n, arch, judge=True):
            "fused_split16_optin_ms": None if split_ms is None else round(split_ms, 1),
            "tuned": {"k": rec["k"], "form": rec["form"], "tune_s": round(t_tune, 1),
                      "calibration_ms": rec["table"]}}
-    norm, _ = libsvx.reduce_f64(state)
+    norm, marg_hi = libsvx.reduce_f64(state, list(range(n - 1, n - 9, -1)))
     out["fused_norm_f64"] = norm
+    if not judge:
+        # bit reversal of the whole state in place (no second state fits beside it at 34 qubits):
+        # the marginals of qubits n-1..n-8 before equal those of qubits 0..7 after (FP64 sums in
+        # another order: within 1e-12), and reversing twice restores the reductions bit for bit
+        rev = list(range(n))[::-1]
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        libsvx.permute(state, state, rev)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        _, marg_lo = libsvx.reduce_f64(state, list(range(8)))
+        libsvx.permute(state, state, rev)
+        norm2, marg_hi2 = libsvx.reduce_f64(state, list(range(n - 1, n - 9, -1)))
+        sweeps = len(libsvx.permute_plan(rev))
+        out["bitrev_in_place"] = {"ms": round(ms, 2), "sweeps": sweeps,
+                                  "gbs_per_sweep": round(sweeps * 2 * (1 << n) * 8 / (ms * 1e-3) / 1e9, 1),
+                                  "marginal_match": float(np.max(np.abs(marg_lo - marg_hi))),
+                                  "twice_restores_bit_exact": bool(norm2 == norm and np.array_equal(marg_hi2, marg_hi))}
     if judge:
         assert fmap_p == fmap_f == list(range(n)), "the layer recipe compiles without relabelling"
         s128 = torch.empty(1 << n, dtype=torch.complex128, device="cuda:0")
@@ -249,9 +406,14 @@ def main():
     ap.add_argument("--no-judge", action="store_true")
     ap.add_argument("--no-fused", action="store_true", help="skip the 32/34-qubit fused-vs-plain section")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix section")
+    ap.add_argument("--no-families", action="store_true", help="skip the 30-qubit families section")
+    ap.add_argument("--no-latency", action="store_true", help="skip the RQC-16 serial-latency section")
+    ap.add_argument("--latency-runs", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-only", action="store_true",
                     help="only the served-mix section (e.g. --serve-seconds 600 for the 10-minute stream)")
+    ap.add_argument("--plain", action="store_true",
+                    help="with --traffic-probe: the workload's passes unfused (every op on the MUX2 kernel)")
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--ladder-probe", action="store_true",
                     help="for rocprofv3 kernel traces: the 30-qubit controlled-phase ladder's complex64 passes "
@@ -302,7 +464,8 @@ def main():
         return
     if a.traffic_probe:
         prog, fmap, st = compiler.compile_gpu(qc)
-        hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
+        with libsvx.options(**(dict(fuse="off") if a.plain else {})):
+            hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
         src = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")
         dst = torch.empty_like(src)
         libsvx.zero_state(src)
@@ -391,10 +554,16 @@ def main():
     torch.cuda.empty_cache()
     fused = None
     if not a.no_fused:
-        fused = {str(nq): fused_section(libsvx, circuit, runtime, torch, nq, arch, judge=(nq == 32 and not a.no_judge))
+        fused = {str(nq): fused_section(libsvx, circuit, runtime, torch, np, nq, arch, judge=(nq == 32 and not a.no_judge))
                  for nq in (32, 34)}
+    families = None if a.no_families else families_section(runtime, circuit, torch, np)
+    latency = None if a.no_latency else latency_section(runtime, circuit, np, runs=a.latency_runs)
+    roof = hbm_roof_section(libsvx, torch)
     served = None if a.no_serve else served_section(serving, torch, seconds=a.serve_seconds)
-    cpu = None if a.no_cpu else _run_cpu_child()
+    cpu = None
+    if not a.no_cpu:
+        cpu = _run_cpu_child()
+        cpu["gpu_same_records"] = gpu_same_records(libsvx, circuit, compiler, torch, np)
     out = {
         "metric": "30q c64 pass GB/s (% of 8 TB/s); fused vs plain time; served circuits/s+p99",
         "value": round(value, 2),
@@ -427,6 +596,9 @@ def main():
         "judge_vs_c128": judge,
         "cp_ladder_30q": ladder,
         "fused_vs_plain": fused,
+        "families_30q": families,
+        "serial_latency_rqc16": latency,
+        "hbm_roof_copy": roof,
         "served_mix": served,
         "cpu_baseline": cpu,
         "device": caps.arch.decode(),

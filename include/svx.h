@@ -158,18 +158,32 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
 
 /* Work of pass `pass` of a prepared segment after classification and pass fusion:
  * out[0] MUX2 register groups, out[1] MUX2 ops, out[2] dense units on 4 qubits, out[3] dense units on
- * 5 qubits, out[4] dense units on the exact FP32 matrix-core path, out[5] pairs of 4-qubit units applied
- * in one LDS round trip, out[6] diagonal tables (runs of diagonal ops applied as one table of phases),
- * out[7] the workgroup-to-tile order of the pass (-2: each XCD takes a contiguous range of tiles,
- * 0: identity; see svx_tune_prepared) (out has 8 entries).  No reference counterpart. */
+ * 5 qubits, out[4] dense units whose entries are all 0, +-1 or +-i (FP32 products exact, so
+ * permutations stay bit-exact), out[5] MUX2 ops applied on the FP32 matrix cores (the MUX2 kernel
+ * alternates general 2x2 ops between the vector units and v_mfma_f32_4x4x1), out[6] diagonal tables
+ * (runs of diagonal ops applied as one table of phases), out[7] the workgroup-to-tile order of the
+ * pass (-2: each XCD takes a contiguous range of tiles, 0: identity; see svx_tune_prepared)
+ * (out has 8 entries).  No reference counterpart. */
 int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out);
 
-/* Qubit permutation perm[new_bit] = old_bit of a 2^n state, out of place (dst != src), bit-exact.
+/* Qubit permutation perm[new_bit] = old_bit of a 2^n state, bit-exact.
  * Replaces the reference's remap paths: model.remap (host/sv/model.py:74-78), the host transpose
  * of local qubits (host/sv/runtime.py:261-268) and the sv_swap bank exchange
- * (host/sv/runtime.py:235-259, kernels/sv/sv_swap.cpp:10) -- any permutation in one HBM sweep. */
+ * (host/sv/runtime.py:235-259, kernels/sv/sv_swap.cpp:10).
+ * Out of place (dst and src disjoint): any permutation in one HBM sweep.
+ * In place (dst == src, or dst NULL -- the reference's sv_swap also exchanges in place): a sequence
+ * of sweeps, each moving only bits inside one 12-bit tile (the low 6 bits plus at most 6 others), so
+ * every workgroup reads and rewrites its own tile; needs no second state.  A permutation of bits
+ * that all lie in one such tile is one sweep; a bit reversal of n >= 12 qubits takes
+ * 1 + ceil((n/2 - 6) / 3) sweeps (svx_permute_plan). */
 int svx_permute(void *dst, const void *src, int32_t n, const int32_t *perm_new_to_old,
                 int32_t dtype, void *stream);
+
+/* The sweeps svx_permute makes for perm: *n_steps = their number (1 out of place); when steps is not
+ * NULL, step i's permutation (position b after it holds what position steps[i n + b] held before)
+ * goes to steps[i n .. i n + n - 1] (at most max_steps steps).  Host only.  No reference counterpart. */
+int svx_permute_plan(int32_t n, const int32_t *perm_new_to_old, int32_t in_place, int32_t *steps,
+                     int32_t max_steps, int32_t *n_steps);
 
 /* The reference swap kernel's contract (kernels/sv/sv_swap.cpp:10): exchange nwords 64-byte words
  * a[offa..offa+nwords) <-> b[offb..).  Ranges must not overlap. */

@@ -13,7 +13,7 @@ from u55c_statevector_30q_amd import circuit as C
 from u55c_statevector_30q_amd import compiler as K
 
 from test_gpu_pass import _mat, _op_words, _rand_state, run_hip
-from _c64 import check_c64, prog_nops, seg_nops
+from _c64 import C_BOUND, amp_error_ratio, check_c64, prog_nops, seg_nops
 
 pytestmark = pytest.mark.gpu
 
@@ -137,8 +137,8 @@ def test_split_fp16_and_fp32_paths(gpu, scale):
 @pytest.mark.parametrize("n,k", [(21, 12), (22, 13)])
 def test_dense_records_double_buffered_kernel(gpu, n, k):
     """The double-buffered kernel's dense instance (XOR-swizzled LDS tile, loader wave + worker
-    waves) against the golden model and bit-identical to the single-buffer kernel (same groups,
-    same FP32 arithmetic)."""
+    waves) against the golden model, and against the single-buffer path (the MUX2 kernel with FP32
+    dense units: the same dense-unit arithmetic, its own MUX2 op forms) within the same bound."""
     from u55c_statevector_30q_amd import libsvx
     rng = np.random.default_rng(5 * n + k)
     seg = dense_segment(rng, n, k, n_passes=2, max_m=4)
@@ -149,4 +149,4 @@ def test_dense_records_double_buffered_kernel(gpu, n, k):
         with libsvx.options(fuse="off", diag_tables=False, mux_engine="valu", kernel=pick):
             got[pick] = run_hip(s0.astype(np.complex64), seg)
         _check(ref, got[pick], (n, k, pick), seg_nops(seg))
-    assert np.array_equal(got["dma"], got["single"])
+    assert amp_error_ratio(got["single"], got["dma"], seg_nops(seg)) <= 2 * C_BOUND

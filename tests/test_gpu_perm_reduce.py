@@ -37,7 +37,9 @@ def test_permute_rejects_bad_input(gpu):
     with pytest.raises(ValueError):
         libsvx.permute(b, a, [0, 1, 2, 3, 3])
     with pytest.raises(ValueError):
-        libsvx.permute(a, a, [0, 1, 2, 3, 4])
+        libsvx.permute(a, a, [0, 1, 2, 3, 5])
+    with pytest.raises(ValueError):
+        libsvx.permute(b, a, [0, 1, 2, 3])
 
 
 def test_swap_words_matches_byte_swap(gpu):

@@ -3,8 +3,8 @@
 Pinned by: closed-form expression values; exact round trips of the gate zoo; the reference's own
 instance files circuits/qasm/*.qasm (written by circuits/generators.py:100-102 through Qiskit's
 qasm2 exporter), which must parse to the same gate lists as the restated generators -- and so
-compile to the same QSV2 stream.  The fixture test reads the reference checkout when it is
-present (this container) and is skipped elsewhere.
+compile to the same QSV2 stream.  The instance files are committed as fixtures under
+tests/golden/qasm/ (copied by tests/golden/make_qasm_fixtures.py).
 """
 import glob
 import math
@@ -16,7 +16,7 @@ import pytest
 
 from u55c_statevector_30q_amd import circuit, compiler, qasm
 
-REF_QASM = "/root/reference/circuits/qasm"
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qasm")
 
 
 @pytest.mark.parametrize("text,val", [
@@ -110,10 +110,13 @@ def _family_of(stem):
     return None
 
 
-_FILES = sorted(glob.glob(os.path.join(REF_QASM, "*.qasm")))
+_FILES = sorted(glob.glob(os.path.join(FIXTURES, "*.qasm")))
 
 
-@pytest.mark.skipif(not _FILES, reason="reference circuits/qasm not present")
+def test_fixtures_present():
+    assert len(_FILES) == 26, _FILES
+
+
 @pytest.mark.parametrize("path", _FILES, ids=[os.path.basename(p) for p in _FILES])
 def test_reference_instances_match_restated_generators(path):
     qc = qasm.load(path)

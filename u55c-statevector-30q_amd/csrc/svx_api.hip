@@ -361,7 +361,9 @@ struct LaunchInfo {
     bool dense;           // pass holds dense (fused-form) groups or tables: the dense instance of k_pass
     bool d4;              // every group a dense 4-qubit unit: the 512-thread dense-only kernel
     bool tunable;         // single-buffer / dense-only launch whose tile order svx_tune_prepared may switch
-    bool mux;             // the FP32 MUX2 kernel k_mux (complex64 passes of MUX2 ops and diagonal tables)
+    bool mux;             // the FP32 MUX2 kernel k_mux (complex64 passes of MUX2 ops, diagonal tables and
+                          // FP32 dense units)
+    bool units;           // the pass holds dense units (not only tables): k_mux's dense instance
     int32_t stats[7];     // MUX2 groups, MUX2 ops, dense 4-qubit, dense 5-qubit, dense exact (FP32),
                           // MUX2 ops on the matrix cores, diagonal tables
 };
@@ -1071,7 +1073,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
         // complex64 passes without dense units (diagonal tables allowed) run on the FP32 MUX2 kernel
         bool has_units = false;
         for (size_t i = 0; i < dense_of.size(); ++i) has_units = has_units || (dense_of[i] >= 0 && !dus[dense_of[i]].diag);
-        const bool use_mux = mode == 1 && dtype == SVX_C64 && !has_units;
+        // the MUX2 kernel: complex64 passes of MUX2 ops and tables, and of FP32 dense units too (the
+        // split-FP16 units need the tile scaling of k_pass)
+        const bool use_mux = mode == 1 && dtype == SVX_C64 && (!has_units || dense_f32);
         const int R = mode == 2 ? 4 : (use_mux ? mux_register_bits(K) : kR);
         PassDev pd{};
         pd.k = k;
@@ -1124,6 +1128,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             li.grid = (uint32_t)li.ntiles;
             li.block = use_mux ? (1u << (K - R)) : kNT;
             li.mux = use_mux;
+            li.units = use_mux && has_units;
             li.lds = pass_lds_bytes(K, rp.m, k, (int)sizeof(T) * 2, false, nbits);
         }
         launches.push_back(li);
@@ -1197,28 +1202,8 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             return off;
         };
         auto is_table = [&](size_t x) { return dense_of[x] >= 0 && dus[dense_of[x]].diag; };
-        if (use_mux) {
-            const int rc = mux_groups<T>(pops, dense_of, dus, K, R, opt.mux == SVX_MUX_AUTO, groups, ops, put_table_regs,
-                                         log2_growth, &launches.back().stats[5]);
-            if (rc) return rc;
-            i = pops.size();
-        }
-        while (i < pops.size()) {
-            if (is_table(i)) {
-                // a table with no MUX2 op before the next dense unit (or the end): its own group, the
-                // LDS tile swept in order (contiguous table reads); beside MUX2 ops it joins their group
-                size_t j = i;
-                while (j < pops.size() && is_table(j)) ++j;
-                if (j == pops.size() || dense_of[j] >= 0) {
-                    GroupDev gd{};
-                    gd.type = 6;
-                    gd.w_off = put_table(dus[dense_of[i]]);
-                    groups.push_back(gd);
-                    ++i;
-                    continue;
-                }
-            }
-            if (dense_of[i] >= 0 && !is_table(i)) {
+        // one dense unit as a group (FP32: three-product or 32x32x2 A operands; split FP16: hi / lo)
+        auto add_dense = [&](size_t i) -> int {
                 const DenseUnit &du = dus[dense_of[i]];
                 const int f0 = du.f, F = std::max(4, f0);
                 GroupDev gd{};
@@ -1261,7 +1246,19 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
                 const int NRT = F == 5 ? 2 : 1;
                 while (wmat.size() % 4) wmat.push_back(0.f);   // 16-byte aligned operand blocks
                 gd.w_off = (int64_t)wmat.size();
-                if (dense_f32 || exact) {
+                gd.exact = exact ? 1 : 0;
+                if (dense_f32 && !exact && F == 4) {
+                    // FP32, three real products (Gauss): wd[(s 4 + t) 64 + l] = W_s[l % 16][4 (l / 16) + t],
+                    // W_0 = Ur + Ui, W_1 = -Ui, W_2 = Ur (A operands of v_mfma_f32_16x16x4f32)
+                    gd.type = 5;
+                    for (int sidx = 0; sidx < 3; ++sidx)
+                        for (int t = 0; t < 4; ++t)
+                            for (int l = 0; l < 64; ++l) {
+                                const int r = l % 16, c = 4 * (l / 16) + t;
+                                const double ur = U(r, c, 0), ui = U(r, c, 1);
+                                wmat.push_back((float)(sidx == 0 ? ur + ui : sidx == 1 ? -ui : ur));
+                            }
+                } else if (dense_f32 || exact) {
                     // FP32 A operand of v_mfma_f32_32x32x2f32: wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32]
                     gd.type = 1;
                     for (int rt = 0; rt < NRT; ++rt)
@@ -1307,6 +1304,47 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
                 }
                 log2_growth += std::log2(std::max(1.0, rs_max));
                 groups.push_back(gd);
+                return 0;
+        };
+        if (use_mux) {
+            // dense units in place; each run of MUX2 ops and tables between them as register groups
+            size_t a0 = 0;
+            while (a0 < pops.size()) {
+                if (dense_of[a0] >= 0 && !is_table(a0)) {
+                    const int rcd = add_dense(a0);
+                    if (rcd) return rcd;
+                    ++a0;
+                    continue;
+                }
+                size_t b0 = a0;
+                while (b0 < pops.size() && !(dense_of[b0] >= 0 && !is_table(b0))) ++b0;
+                const std::vector<OpDev<T>> sub(pops.begin() + a0, pops.begin() + b0);
+                const std::vector<int> sub_dense(dense_of.begin() + a0, dense_of.begin() + b0);
+                const int rc = mux_groups<T>(sub, sub_dense, dus, K, R, opt.mux == SVX_MUX_AUTO, groups, ops,
+                                             put_table_regs, log2_growth, &launches.back().stats[5]);
+                if (rc) return rc;
+                a0 = b0;
+            }
+            i = pops.size();
+        }
+        while (i < pops.size()) {
+            if (is_table(i)) {
+                // a table with no MUX2 op before the next dense unit (or the end): its own group, the
+                // LDS tile swept in order (contiguous table reads); beside MUX2 ops it joins their group
+                size_t j = i;
+                while (j < pops.size() && is_table(j)) ++j;
+                if (j == pops.size() || dense_of[j] >= 0) {
+                    GroupDev gd{};
+                    gd.type = 6;
+                    gd.w_off = put_table(dus[dense_of[i]]);
+                    groups.push_back(gd);
+                    ++i;
+                    continue;
+                }
+            }
+            if (dense_of[i] >= 0 && !is_table(i)) {
+                const int rcd = add_dense(i);
+                if (rcd) return rcd;
                 ++i;
                 continue;
             }
@@ -1404,7 +1442,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
                 lr.stats[6] += 1;
             } else {
                 lr.stats[G.f == 5 ? 3 : 2] += 1;
-                lr.stats[4] += G.type == 1;
+                lr.stats[4] += G.exact;
             }
         }
         passes.push_back(pd);
@@ -1587,8 +1625,8 @@ hipError_t launch_one(void *state, const Prepared *pr, const LaunchInfo &li, hip
         return launch_pass_dma(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
                                li.ntiles, li.grid, li.lds, s);
     if (li.mux)
-        return launch_mux(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd, li.grid,
-                          li.lds, s);
+        return launch_mux(li.K, li.units, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
+                          li.grid, li.lds, s);
     if (pr->dtype == SVX_C64)
         return launch_pass_c64(li.K, li.small, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off),
                                wd, li.grid, li.lds, s);
@@ -1618,10 +1656,27 @@ int get_prepared(int64_t handle, Prepared **out) {
 }
 
 // ------------------------------------------------------------------------------------------ permute
+// Permutation descriptors live in a bounded cache keyed by [device, n, perm...].  Each entry records
+// an event after its launches; an entry is evicted (least recently used first) only once that event
+// has completed, and entries used under stream capture are kept (a captured graph reads them later).
+struct PermEntry {
+    void *dev = nullptr;
+    uint32_t grid = 0;
+    hipEvent_t ev = nullptr;
+    uint64_t lru = 0;
+    bool pinned = false;
+};
+constexpr size_t kPermCacheMax = 64;
 std::mutex g_perm_mu;
-std::map<std::vector<int32_t>, std::pair<void *, uint32_t>> g_perm_cache;   // [device, n, perm...] -> (PermDev, grid)
+std::map<std::vector<int32_t>, PermEntry> g_perm_cache;
+uint64_t g_perm_tick = 0;
 
-int build_perm(int n, const int32_t *perm, PermDev *pd, uint32_t *grid) {
+// Tile of a permutation sweep: v = min(n, 12) bits holding new bits 0..c-1 (c = min(6, n)) and the new
+// positions of old bits 0..c-1, so reads and writes both run over 2^c contiguous amplitudes; the
+// other n - v bits are fixed per workgroup.  `tile` (optional) names further bits the tile must hold:
+// an in-place step passes the bits it moves, and since the step maps that set onto itself and fixes
+// every other bit, each workgroup reads and writes the same 2^v amplitudes.
+int build_perm(int n, const int32_t *perm, PermDev *pd, uint32_t *grid, const std::vector<int> *tile = nullptr) {
     std::vector<int> inv(n, -1);
     for (int b = 0; b < n; ++b) {
         if (perm[b] < 0 || perm[b] >= n || inv[perm[b]] != -1) return fail(SVX_E_ARG, "perm is not a permutation of 0..n-1");
@@ -1639,6 +1694,8 @@ int build_perm(int n, const int32_t *perm, PermDev *pd, uint32_t *grid) {
     std::vector<char> inV(n, 0);
     for (int b = 0; b < c; ++b) inV[b] = 1;
     for (int o = 0; o < c; ++o) inV[inv[o]] = 1;
+    if (tile)
+        for (int b : *tile) inV[b] = 1;
     int cnt = 0;
     for (int b = 0; b < n; ++b) cnt += inV[b];
     for (int b = 0; b < n && cnt < v; ++b)
@@ -1885,7 +1942,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
             it.gd = reinterpret_cast<const GroupDev *>(base + pr->group_off);
             it.od = base + pr->op_off;
             it.wd = li.dense ? reinterpret_cast<const float *>(base + pr->w_off) : nullptr;
-            Cls &c = cls[li.K * 8 + (li.dense ? 1 : 0) + (li.d4 ? 2 : 0) + (li.mux ? 4 : 0)];
+            Cls &c = cls[li.K * 16 + (li.dense ? 1 : 0) + (li.d4 ? 2 : 0) + (li.mux ? 4 : 0) + (li.units ? 8 : 0)];
             it.tile0 = c.tiles;
             c.tiles += li.ntiles;
             c.lds = std::max(c.lds, li.lds);
@@ -1901,7 +1958,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
     }
     for (auto &kv : cls) {
         Cls &c = kv.second;
-        const int bK = kv.first / 8;
+        const int bK = kv.first / 16;
         const bool bmux = (kv.first & 4) != 0;
         const uint64_t threads = bmux ? (uint64_t(1) << (bK - mux_register_bits(bK))) : ((kv.first & 2) ? kNTD4 : kNT);
         if (c.tiles * threads > 0xffffffffull)
@@ -1913,7 +1970,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
         BatchItem *dv = ds->batch_dev + (size_t)slot * DeviceState::kBatchCap;
         std::memcpy(hs, c.items.data(), c.items.size() * sizeof(BatchItem));
         HIPCHK(hipMemcpyAsync(dv, hs, c.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, s));
-        const hipError_t e = bmux ? launch_mux_batch(bK, dv, (int)c.items.size(), (uint32_t)c.tiles, c.lds, s)
+        const hipError_t e = bmux ? launch_mux_batch(bK, (kv.first & 8) != 0, dv, (int)c.items.size(), (uint32_t)c.tiles, c.lds, s)
                                   : launch_pass_batch(bK, kv.first & 1, (kv.first & 2) != 0, dv, (int)c.items.size(),
                                                       (uint32_t)c.tiles, c.lds, s);
         if (e != hipSuccess) return hipfail(e, "batched pass launch");
@@ -2029,37 +2086,213 @@ int svx_engine(void *st, int64_t ins_off, void *stream) {
     return svx_run_segment(st, int64_t(1) << L, 1, SVX_C64, words.data(), (int64_t)words.size(), stream);
 }
 
+// In-place plan of perm[new] = old: steps s_1 .. s_S, each a permutation that moves only bits of a set
+// M_i with at most 6 bits outside the low 6 (so M_i fits one 12-bit sweep tile, `tile`), applied in
+// order (position b after step i holds what position s_i[b] held before it).  Position b needs the
+// bit now at need(b); the cycles of `need` are taken whole while the tile has room (smallest first),
+// else as a segment b_1 .. b_j of one cycle, which places j - 1 bits and leaves the cycle j - 1
+// shorter.  Each step is one in-place sweep; at most n steps.
+static int plan_inplace(int n, const int32_t *perm, std::vector<std::vector<int32_t>> *steps,
+                        std::vector<std::vector<int>> *tiles) {
+    const int c = std::min(6, n), budget = std::min(12, n) - c;
+    std::vector<int> cur(n), pos(n);   // cur[b]: original bit at position b; pos: its inverse
+    for (int b = 0; b < n; ++b) cur[b] = pos[b] = b;
+    for (int guard = 0; guard <= n; ++guard) {
+        std::vector<int> need(n);
+        bool done = true;
+        for (int b = 0; b < n; ++b) {
+            need[b] = pos[perm[b]];
+            done = done && need[b] == b;
+        }
+        if (done) return 0;
+        // cycles b -> need(b) -> need(need(b)) ... of the positions not yet in place
+        std::vector<char> seen(n, 0);
+        std::vector<std::vector<int>> cyc;
+        for (int b = 0; b < n; ++b) {
+            if (seen[b] || need[b] == b) continue;
+            std::vector<int> cy;
+            for (int x = b; !seen[x]; x = need[x]) {
+                seen[x] = 1;
+                cy.push_back(x);
+            }
+            cyc.push_back(cy);
+        }
+        auto high = [&](const std::vector<int> &v) {
+            int h = 0;
+            for (int x : v) h += x >= c;
+            return h;
+        };
+        std::sort(cyc.begin(), cyc.end(), [&](const std::vector<int> &a, const std::vector<int> &b) {
+            return high(a) < high(b);
+        });
+        std::vector<char> inM(n, 0);
+        int used = 0;
+        bool any = false;
+        for (const auto &cy : cyc) {   // whole cycles that fit
+            const int h = high(cy);
+            if (used + h > budget) continue;
+            for (int x : cy) inM[x] = 1;
+            used += h;
+            any = true;
+        }
+        if (!any) {   // the longest segment of the first cycle within the budget
+            const auto &cy = cyc[0];
+            const int L = (int)cy.size();
+            int best_len = 0, best_start = 0;
+            for (int st = 0; st < L; ++st) {
+                int h = 0, len = 0;
+                while (len < L) {
+                    const int hx = cy[(st + len) % L] >= c;
+                    if (h + hx > budget) break;
+                    h += hx;
+                    ++len;
+                }
+                if (len > best_len) {
+                    best_len = len;
+                    best_start = st;
+                }
+            }
+            if (best_len < 2) return fail(SVX_E_ARG, "internal: in-place permutation plan");
+            for (int t = 0; t < best_len; ++t) inM[cy[(best_start + t) % L]] = 1;
+        }
+        // the step: b in M takes need(b) when that lies in M; the rest of M takes the unclaimed bits
+        std::vector<int32_t> stp(n);
+        std::vector<int> open, spare;
+        std::vector<char> claimed(n, 0);
+        for (int b = 0; b < n; ++b) {
+            stp[b] = b;
+            if (!inM[b]) continue;
+            if (inM[need[b]]) {
+                stp[b] = need[b];
+                claimed[need[b]] = 1;
+            } else {
+                open.push_back(b);
+            }
+        }
+        for (int b = 0; b < n; ++b)
+            if (inM[b] && !claimed[b]) spare.push_back(b);
+        if (open.size() != spare.size()) return fail(SVX_E_ARG, "internal: in-place permutation step");
+        for (size_t t = 0; t < open.size(); ++t) stp[open[t]] = spare[t];
+        std::vector<int> tile;
+        for (int b = 0; b < n; ++b)
+            if (inM[b]) tile.push_back(b);
+        std::vector<int> nc(n);
+        for (int b = 0; b < n; ++b) nc[b] = cur[stp[b]];
+        cur = nc;
+        for (int b = 0; b < n; ++b) pos[cur[b]] = b;
+        steps->push_back(stp);
+        tiles->push_back(tile);
+    }
+    return fail(SVX_E_ARG, "internal: in-place permutation did not converge");
+}
+
+// A cached permutation descriptor (built and uploaded on first use), evicting the least recently
+// used completed entry when the cache is full.
+static int perm_entry(DeviceState *ds, int n, const int32_t *perm, const std::vector<int> *tile, hipStream_t s,
+                      PermEntry **out) {
+    std::vector<int32_t> key{ds->caps.device, n, tile ? 1 : 0};
+    key.insert(key.end(), perm, perm + n);
+    auto it = g_perm_cache.find(key);
+    if (it == g_perm_cache.end()) {
+        PermDev pd;
+        uint32_t grid = 0;
+        int rc = build_perm(n, perm, &pd, &grid, tile);
+        if (rc) return rc;
+        while (g_perm_cache.size() >= kPermCacheMax) {
+            auto victim = g_perm_cache.end();
+            for (auto jt = g_perm_cache.begin(); jt != g_perm_cache.end(); ++jt)
+                if (!jt->second.pinned && jt->first[0] == ds->caps.device &&
+                    (victim == g_perm_cache.end() || jt->second.lru < victim->second.lru))
+                    victim = jt;
+            if (victim == g_perm_cache.end()) break;   // every entry is pinned by a captured graph
+            HIPCHK(hipEventSynchronize(victim->second.ev));
+            HIPCHK(hipFree(victim->second.dev));
+            (void)hipEventDestroy(victim->second.ev);
+            g_perm_cache.erase(victim);
+        }
+        PermEntry e;
+        e.grid = grid;
+        HIPCHK(hipMalloc(&e.dev, sizeof(PermDev)));
+        HIPCHK(hipMemcpy(e.dev, &pd, sizeof(PermDev), hipMemcpyHostToDevice));
+        HIPCHK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+        it = g_perm_cache.emplace(key, e).first;
+    }
+    it->second.lru = ++g_perm_tick;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) it->second.pinned = true;
+    *out = &it->second;
+    return 0;
+}
+
+static int perm_launch(int dtype, void *dst, const void *src, int n, PermEntry *pe, hipStream_t s) {
+    const int v = n <= 12 ? n : 12;
+    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 4)) * (dtype == SVX_C64 ? 8 : 16);
+    hipError_t e = launch_permute(dtype, dst, src, (const PermDev *)pe->dev, pe->grid, lds, s);
+    if (e != hipSuccess) return hipfail(e, "permute launch");
+    if (!pe->pinned) (void)hipEventRecord(pe->ev, s);
+    return 0;
+}
+
 int svx_permute(void *dst, const void *src, int32_t n, const int32_t *perm, int32_t dtype, void *stream) {
-    if (!dst || !src || !perm) return fail(SVX_E_ARG, "null pointer");
-    if (dst == src) return fail(SVX_E_ARG, "svx_permute is out of place: dst must differ from src");
+    if (!src || !perm) return fail(SVX_E_ARG, "null pointer");
+    if (!dst) dst = const_cast<void *>(src);   // NULL destination: in place
     if (n < 1 || n > 40) return fail(SVX_E_ARG, "n out of range");
     if (dtype != SVX_C64 && dtype != SVX_C128) return fail(SVX_E_ARG, "bad dtype");
+    const size_t bytes = (size_t(1) << n) * (dtype == SVX_C64 ? 8 : 16);
+    const char *pd = static_cast<const char *>(dst), *ps = static_cast<const char *>(src);
+    if (dst != src && pd < ps + bytes && ps < pd + bytes) return fail(SVX_E_ARG, "dst and src overlap partially");
+    {
+        std::vector<int> seen(n, 0);
+        for (int b = 0; b < n; ++b) {
+            if (perm[b] < 0 || perm[b] >= n || seen[perm[b]]) return fail(SVX_E_ARG, "perm is not a permutation of 0..n-1");
+            seen[perm[b]] = 1;
+        }
+    }
     DeviceState *ds = nullptr;
     int rc = get_ds(&ds);
     if (rc) return rc;
-    std::vector<int32_t> key{ds->caps.device, n};
-    key.insert(key.end(), perm, perm + n);
-    void *pdv = nullptr;
-    uint32_t grid = 0;
-    {
-        std::lock_guard<std::mutex> lk(g_perm_mu);
-        auto it = g_perm_cache.find(key);
-        if (it != g_perm_cache.end()) {
-            pdv = it->second.first;
-            grid = it->second.second;
-        } else {
-            PermDev pd;
-            rc = build_perm(n, perm, &pd, &grid);
-            if (rc) return rc;
-            HIPCHK(hipMalloc(&pdv, sizeof(PermDev)));
-            HIPCHK(hipMemcpy(pdv, &pd, sizeof(PermDev), hipMemcpyHostToDevice));
-            g_perm_cache[key] = {pdv, grid};
-        }
+    const hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_perm_mu);
+    if (dst != src) {
+        PermEntry *pe = nullptr;
+        rc = perm_entry(ds, n, perm, nullptr, s, &pe);
+        return rc ? rc : perm_launch(dtype, dst, src, n, pe, s);
     }
-    const int v = n <= 12 ? n : 12;
-    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 4)) * (dtype == SVX_C64 ? 8 : 16);
-    hipError_t e = launch_permute(dtype, dst, src, (const PermDev *)pdv, grid, lds, (hipStream_t)stream);
-    if (e != hipSuccess) return hipfail(e, "permute launch");
+    std::vector<std::vector<int32_t>> steps;
+    std::vector<std::vector<int>> tiles;
+    rc = plan_inplace(n, perm, &steps, &tiles);
+    if (rc) return rc;
+    for (size_t i = 0; i < steps.size(); ++i) {
+        PermEntry *pe = nullptr;
+        rc = perm_entry(ds, n, steps[i].data(), &tiles[i], s, &pe);
+        if (rc) return rc;
+        rc = perm_launch(dtype, dst, dst, n, pe, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int svx_permute_plan(int32_t n, const int32_t *perm, int32_t in_place, int32_t *steps, int32_t max_steps,
+                     int32_t *n_steps) {
+    if (!perm || !n_steps || n < 1 || n > 40 || max_steps < 0) return fail(SVX_E_ARG, "bad permute-plan arguments");
+    std::vector<int> seen(n, 0);
+    for (int b = 0; b < n; ++b) {
+        if (perm[b] < 0 || perm[b] >= n || seen[perm[b]]) return fail(SVX_E_ARG, "perm is not a permutation of 0..n-1");
+        seen[perm[b]] = 1;
+    }
+    std::vector<std::vector<int32_t>> st;
+    std::vector<std::vector<int>> tiles;
+    if (in_place) {
+        const int rc = plan_inplace(n, perm, &st, &tiles);
+        if (rc) return rc;
+    } else {
+        st.emplace_back(perm, perm + n);
+    }
+    *n_steps = (int32_t)st.size();
+    if (steps) {
+        if ((int32_t)st.size() > max_steps) return fail(SVX_E_ARG, "steps buffer too small");
+        for (size_t i = 0; i < st.size(); ++i) std::memcpy(steps + i * n, st[i].data(), n * sizeof(int32_t));
+    }
     return 0;
 }
 

@@ -165,4 +165,174 @@ __device__ __forceinline__ uint64_t tile_order(uint64_t b, const PassDev *pd) {
     return ((b << rr) | (b >> (tbits - rr))) & mask;
 }
 
+// The LDS tile layout is a template choice: padded (byte address 8 lpad(e), additive over disjoint
+// bits: the single-buffer kernel) or XOR-swizzled (swz(e), XOR-linear: the double-buffered kernel).
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t swz(uint32_t e) {
+    return (((e >> 1) ^ ((e >> 5) & 15u)) << 4) | ((e & 1u) << 3);
+}
+
+template <bool SWZ>
+struct TileAddr {
+    __device__ __forceinline__ static uint32_t of(uint32_t e) { return SWZ ? swz(e) : 8u * lpad(e); }
+    __device__ __forceinline__ static uint32_t comb(uint32_t a, uint32_t b) { return SWZ ? (a ^ b) : (a + b); }
+};
+
+// byte offsets of the 2^F components of one column, folded from the per-bit offsets
+template <int F, bool SWZ>
+struct ColOff {
+    uint32_t lq[F];
+    __device__ __forceinline__ void init(const GroupDev &G) {
+#pragma unroll
+        for (int i = 0; i < F; ++i) lq[i] = TileAddr<SWZ>::of(1u << G.dq[i]);
+    }
+    __device__ __forceinline__ uint32_t operator()(uint32_t c) const {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+            if ((c >> i) & 1u) o = TileAddr<SWZ>::comb(o, lq[i]);
+        return o;
+    }
+};
+
+__device__ __forceinline__ C2<float> ldc(const unsigned char *buf, uint32_t a) {
+    return *reinterpret_cast<const C2<float> *>(buf + a);
+}
+__device__ __forceinline__ void stc(unsigned char *buf, uint32_t a, C2<float> v) {
+    *reinterpret_cast<C2<float> *>(buf + a) = v;
+}
+
+// FP32 matrix cores (exact for 0 / +-1 matrices): v_mfma_f32_32x32x2f32, half h = l / 32 feeds K-rows
+// 2s + h; W arrives in A-operand order wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32].
+// NWAVE wavefronts (wt = 0 .. 64 NWAVE - 1) share the columns.
+template <int K, int F, int NWAVE, bool SWZ>
+__device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                            uint32_t wt) {
+    constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
+    constexpr int NS = 1 << F;                     // K-steps of 2 real rows
+    constexpr int NX = NS / 2;                     // complex inputs per lane
+    constexpr uint32_t NB = (1u << (K - F)) / 32;  // 32-column batches per tile
+    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
+    const float *w = wd + G.w_off;
+    float wa[NRT][NS];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wa[rt][s] = w[(rt * NS + s) * 64 + lane];
+    ColOff<F, SWZ> coff;
+    coff.init(G);
+    const uint32_t hoff = h ? coff.lq[0] : 0u;
+    for (uint32_t b = wave; b < NB; b += NWAVE) {
+        uint32_t base = b * 32u + j;
+#pragma unroll
+        for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
+        const uint32_t pb = TileAddr<SWZ>::of(base);
+        C2<float> x[NX];
+#pragma unroll
+        for (int s = 0; s < NX; ++s) x[s] = ldc(buf, TileAddr<SWZ>::comb(pb, TileAddr<SWZ>::comb(coff(2u * s), hoff)));
+        f16v acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+            acc[rt] = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const float bv = s < NX ? x[s].x : x[s - NX].y;
+                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[rt][s], bv, acc[rt], 0, 0, 0);
+            }
+        }
+        if (F == 4) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[0][v + 8]});
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
+                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[NRT - 1][v]});
+            }
+        }
+    }
+}
+
+typedef float f4acc __attribute__((ext_vector_type(4)));
+
+// FP32 matrix cores, 4-qubit units as three real 16 x 16 products (Gauss's complex product): with
+// W = Wr + i Wi and a column x = p + i q, k1 = (Wr + Wi) p, Re y = k1 - Wi (p + q),
+// Im y = k1 + Wr (q - p) -- 48 multiply-adds per amplitude instead of 64.  v_mfma_f32_16x16x4f32:
+// lane l holds A[l % 16][l / 16] and B[l / 16][l % 16], and D[4 (l / 16) + v][l % 16] in register v.
+// K step t, lane quarter g = l / 16 carries component 4 g + t, so a lane reads components
+// 4 g .. 4 g + 3 of column l % 16 and writes the same four outputs (no other lane touches them).
+// Operands: wd[(s 4 + t) 64 + l] = W_s[l % 16][4 (l / 16) + t] with W_0 = Wr + Wi, W_1 = -Wi, W_2 = Wr.
+// Each wavefront step carries two 16-column blocks, two independent MFMA chains.
+template <int K, int NWAVE, bool SWZ>
+__device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                             uint32_t wt) {
+    constexpr uint32_t NB = (1u << (K - 4)) / 16;   // 16-column blocks per tile
+    const uint32_t lane = wt & 63u, wave = wt >> 6, g = lane >> 4, j = lane & 15u;
+    const float *w = wd + G.w_off;
+    float w0[4], w1[4], w2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        w0[t] = w[t * 64 + lane];
+        w1[t] = w[(4 + t) * 64 + lane];
+        w2[t] = w[(8 + t) * 64 + lane];
+    }
+    ColOff<4, SWZ> coff;
+    coff.init(G);
+    uint32_t co[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) co[t] = coff(4u * g + t);
+    for (uint32_t b = wave; b < NB; b += 2 * NWAVE) {
+        uint32_t pb[2];
+        C2<float> x[2][4];
+        // the second block of a step, when the tile has one (else block b again, not stored)
+        const bool two = b + NWAVE < NB;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint32_t base = (b + (h && two ? NWAVE : 0)) * 16u + j;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) base = ins0_32(base, G.dqs[i]);
+            pb[h] = TileAddr<SWZ>::of(base);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[h][t] = ldc(buf, TileAddr<SWZ>::comb(pb[h], co[t]));
+        }
+        f4acc k1[2], re[2], im[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            k1[h] = f4acc{0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) k1[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w0[t], x[h][t].x, k1[h], 0, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            re[h] = k1[h];
+            im[h] = k1[h];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                re[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w1[t], x[h][t].x + x[h][t].y, re[h], 0, 0, 0);
+                im[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w2[t], x[h][t].y - x[h][t].x, im[h], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) stc(buf, TileAddr<SWZ>::comb(pb[0], co[v]), C2<float>{re[0][v], im[0][v]});
+        if (two)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) stc(buf, TileAddr<SWZ>::comb(pb[1], co[v]), C2<float>{re[1][v], im[1][v]});
+    }
+}
+
+
+// FP32 dense groups (type 1: 32x32x2 real form; type 5: 4-qubit three-product form) by NWAVE
+// wavefronts, for kernels without the split-FP16 paths
+template <int K, int NWAVE, bool SWZ>
+__device__ __forceinline__ void dense_group_f32(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
+                                                uint32_t wt) {
+    if (G.type == 5) dense_gauss4<K, NWAVE, SWZ>(buf, G, wd, wt);
+    else if (G.f == 5) dense_apply<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
+    else dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
+}
+
 }  // namespace svx

@@ -63,7 +63,9 @@ struct GroupDev {
     int32_t nops, op0;
     int32_t b[kR];               // workgroup-tile bit held by register bit q
     int32_t bs[kR];              // b sorted ascending (slot deposit)
-    int32_t type;                // 0: MUX2 register group; 1: dense op, FP32 MFMA; 2: dense op, small-state
+    int32_t type;                // 0: MUX2 register group; 1: dense op, FP32 MFMA (32x32x2, complex as a
+                                 // 2^(f+1) real matrix); 5: 4-qubit dense op, FP32 MFMA (16x16x4, three
+                                 // real products); 2: dense op, small-state
                                  // kernel; 3: dense op, split-FP16 MFMA, per-column scaling; 4: the same
                                  // on a tile pre-scaled by the pass kernel (PassDev::tscale);
                                  // 6: diagonal table on its own (no MUX2 op beside it), 2^k complex64
@@ -72,7 +74,7 @@ struct GroupDev {
     int32_t dq[5];               // dense: workgroup-tile bit of the op's qubit i (matrix index bit i)
     int32_t dqs[5];              // dq sorted ascending (column deposit)
     int32_t wexp;                // split-FP16 dense: the matrix was scaled by 2^wexp before the split
-    int32_t pad1;
+    int32_t exact;               // dense: entries all in {0, +-1, +-i} (FP32 products exact; statistics)
     int64_t w_off;               // dense: float offset of the matrix (type 1: MFMA A-operand order;
                                  // type 2: row-major re, im)
 };
@@ -124,9 +126,10 @@ hipError_t set_kernel_attributes();
 constexpr int kMuxR13 = SVX_MUX_R13;   // register bits at K = 13 (4: 512 threads, 5: 256 threads)
 int mux_register_bits(int K);
 hipError_t set_mux_attributes();
-hipError_t launch_mux(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+hipError_t launch_mux(int K, bool dense, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
                       const float *wd, uint32_t grid, size_t lds, hipStream_t s);
-hipError_t launch_mux_batch(int K, const BatchItem *items, int n_items, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_mux_batch(int K, bool dense, const BatchItem *items, int n_items, uint32_t grid, size_t lds,
+                            hipStream_t s);
 // double-buffered complex64 pass kernel (one QSV2 tile of 2^K per LDS buffer, K in {12, 13});
 // register groups of 4 tile bits
 size_t pass_dma_lds_bytes(int K, int m);

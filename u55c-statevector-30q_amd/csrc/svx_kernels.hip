@@ -287,98 +287,7 @@ __device__ __forceinline__ int load_tile_scaled(C2<float> *__restrict__ st, C2<f
 // Im x_0..).  A wavefront handles 32 columns at a time (lane l owns column l % 32); the accumulator
 // layout of a 32x32 output block is row i = 8 (v / 4) + 4 (l / 32) + v % 4 for accumulator v.
 //
-// The LDS tile layout is a template choice: padded (byte address 8 lpad(e), additive over disjoint
-// bits: the single-buffer kernel) or XOR-swizzled (swz(e), XOR-linear: the double-buffered kernel).
-typedef float f16v __attribute__((ext_vector_type(16)));
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ uint32_t swz(uint32_t e) {
-    return (((e >> 1) ^ ((e >> 5) & 15u)) << 4) | ((e & 1u) << 3);
-}
-
-template <bool SWZ>
-struct TileAddr {
-    __device__ __forceinline__ static uint32_t of(uint32_t e) { return SWZ ? swz(e) : 8u * lpad(e); }
-    __device__ __forceinline__ static uint32_t comb(uint32_t a, uint32_t b) { return SWZ ? (a ^ b) : (a + b); }
-};
-
-// byte offsets of the 2^F components of one column, folded from the per-bit offsets
-template <int F, bool SWZ>
-struct ColOff {
-    uint32_t lq[F];
-    __device__ __forceinline__ void init(const GroupDev &G) {
-#pragma unroll
-        for (int i = 0; i < F; ++i) lq[i] = TileAddr<SWZ>::of(1u << G.dq[i]);
-    }
-    __device__ __forceinline__ uint32_t operator()(uint32_t c) const {
-        uint32_t o = 0;
-#pragma unroll
-        for (int i = 0; i < F; ++i)
-            if ((c >> i) & 1u) o = TileAddr<SWZ>::comb(o, lq[i]);
-        return o;
-    }
-};
-
-__device__ __forceinline__ C2<float> ldc(const unsigned char *buf, uint32_t a) {
-    return *reinterpret_cast<const C2<float> *>(buf + a);
-}
-__device__ __forceinline__ void stc(unsigned char *buf, uint32_t a, C2<float> v) {
-    *reinterpret_cast<C2<float> *>(buf + a) = v;
-}
-
-// FP32 matrix cores (exact for 0 / +-1 matrices): v_mfma_f32_32x32x2f32, half h = l / 32 feeds K-rows
-// 2s + h; W arrives in A-operand order wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32].
-// NWAVE wavefronts (wt = 0 .. 64 NWAVE - 1) share the columns.
-template <int K, int F, int NWAVE, bool SWZ>
-__device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
-                                            uint32_t wt) {
-    constexpr int NRT = F == 5 ? 2 : 1;            // 32-row output tiles
-    constexpr int NS = 1 << F;                     // K-steps of 2 real rows
-    constexpr int NX = NS / 2;                     // complex inputs per lane
-    constexpr uint32_t NB = (1u << (K - F)) / 32;  // 32-column batches per tile
-    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
-    const float *w = wd + G.w_off;
-    float wa[NRT][NS];
-#pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-        for (int s = 0; s < NS; ++s) wa[rt][s] = w[(rt * NS + s) * 64 + lane];
-    ColOff<F, SWZ> coff;
-    coff.init(G);
-    const uint32_t hoff = h ? coff.lq[0] : 0u;
-    for (uint32_t b = wave; b < NB; b += NWAVE) {
-        uint32_t base = b * 32u + j;
-#pragma unroll
-        for (int i = 0; i < F; ++i) base = ins0_32(base, G.dqs[i]);
-        const uint32_t pb = TileAddr<SWZ>::of(base);
-        C2<float> x[NX];
-#pragma unroll
-        for (int s = 0; s < NX; ++s) x[s] = ldc(buf, TileAddr<SWZ>::comb(pb, TileAddr<SWZ>::comb(coff(2u * s), hoff)));
-        f16v acc[NRT];
-#pragma unroll
-        for (int rt = 0; rt < NRT; ++rt) {
-            acc[rt] = f16v{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const float bv = s < NX ? x[s].x : x[s - NX].y;
-                acc[rt] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[rt][s], bv, acc[rt], 0, 0, 0);
-            }
-        }
-        if (F == 4) {
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[0][v + 8]});
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[NRT - 1][v]});
-            }
-        }
-    }
-}
 
 // Split-FP16 matrix cores: W (scaled on the host by 2^wexp -- 0 for entries within [2^-4, 2], else
 // so its largest entry is below 2^14 -- and split W 2^wexp = Wh + Wl) times a column block x scaled
@@ -672,7 +581,9 @@ __device__ __forceinline__ bool dense4_h128(unsigned char *buf, const GroupDev &
 template <int K, int NWAVE, bool SWZ>
 __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                             uint32_t wt) {
-    if (G.type == 1) {
+    if (G.type == 5) {
+        dense_gauss4<K, NWAVE, SWZ>(buf, G, wd, wt);
+    } else if (G.type == 1) {
         if (G.f == 5) dense_apply<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
         else dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
     } else if (G.type == 4 && !SWZ) {
@@ -688,7 +599,9 @@ __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &
 template <int K, int NWAVE, bool SWZ = false>
 __device__ __forceinline__ void dense_group4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                              uint32_t wt) {
-    if (G.type == 1) {
+    if (G.type == 5) {
+        dense_gauss4<K, NWAVE, SWZ>(buf, G, wd, wt);
+    } else if (G.type == 1) {
         dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
     } else if (G.type == 4) {
         if (!SWZ || !dense4_h128<K, NWAVE, true>(buf, G, wd, wt)) dense_apply_h<K, 4, NWAVE, SWZ, true>(buf, G, wd, wt);
@@ -1408,16 +1321,31 @@ __global__ void k_convert(C2<TD> *__restrict__ d, const C2<TS> *__restrict__ s, 
         d[i] = C2<TD>{TD(s[i].x), TD(s[i].y)};
 }
 
+// Same-type copy at the HBM roof: one 16-byte non-temporal load and store per thread and launch
+// round, one-shot workgroups up to 2^22 of them (measured on 8 GiB: one access per thread in flight
+// beats 2..16 per thread and grid-stride loops); `tail` copies a last odd 8-byte element.
+__global__ __launch_bounds__(256) void k_copy16(f4v *__restrict__ d, const f4v *__restrict__ s, int64_t n16,
+                                                uint2 *__restrict__ dt, const uint2 *__restrict__ st, int tail) {
+    const int64_t stride = int64_t(gridDim.x) * 256;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
+    if (tail && blockIdx.x == 0 && threadIdx.x == 0) *dt = *st;
+}
+
 hipError_t launch_convert(void *dst, int dt_dst, const void *src, int dt_src, int64_t n, hipStream_t s) {
     const uint32_t g = grid_for(n, 8);
-    if (dt_dst == 0 && dt_src == 0)
-        hipLaunchKernelGGL((k_convert<float, float>), dim3(g), dim3(256), 0, s, (C2<float> *)dst, (const C2<float> *)src, n);
-    else if (dt_dst == 0)
+    if (dt_dst == dt_src) {
+        const int64_t bytes = n * (dt_src == 0 ? 8 : 16), n16 = bytes / 16;
+        const int tail = int(bytes & 15) ? 1 : 0;
+        int64_t gb = (n16 + 255) / 256;
+        if (gb > (int64_t(1) << 22)) gb = int64_t(1) << 22;
+        if (gb < 1) gb = 1;
+        hipLaunchKernelGGL(k_copy16, dim3(uint32_t(gb)), dim3(256), 0, s, (f4v *)dst, (const f4v *)src, n16,
+                           (uint2 *)((char *)dst + n16 * 16), (const uint2 *)((const char *)src + n16 * 16), tail);
+    } else if (dt_dst == 0)
         hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(256), 0, s, (C2<float> *)dst, (const C2<double> *)src, n);
-    else if (dt_src == 0)
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, s, (C2<double> *)dst, (const C2<float> *)src, n);
     else
-        hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(256), 0, s, (C2<double> *)dst, (const C2<double> *)src, n);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, s, (C2<double> *)dst, (const C2<float> *)src, n);
     return hipGetLastError();
 }
 

@@ -332,7 +332,9 @@ __device__ __forceinline__ void apply(const Rec &rc, const float (&x)[2 << R], f
 }  // namespace mux
 
 // One workgroup tile of a MUX2 pass (k_mux and its batched form): NT = 2^(K-R) threads, one slot each.
-template <int K, int R>
+// DN: the pass also holds FP32 dense units (group types 1 and 5), applied by all NT / 64 wavefronts
+// on the padded LDS tile between the register groups.
+template <int K, int R, bool DN>
 __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                          const GroupDev *__restrict__ gd, const OpDev<float> *__restrict__ od,
                                          const float *__restrict__ wd, uint64_t tile_id, unsigned char *smem) {
@@ -368,6 +370,13 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
             diag_table<K, NT>(tile, wd + G.w_off, k, tid);
             __syncthreads();
             continue;
+        }
+        if constexpr (DN) {
+            if (G.type != 0) {
+                dense_group_f32<K, NT / 64, false>(smem, G, wd, tid);
+                __syncthreads();
+                continue;
+            }
         }
         uint32_t poff[NR];   // padded LDS offset of register r (disjoint bits: pad(a|b) = pad(a)+pad(b))
         poff[0] = 0;
@@ -409,15 +418,16 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
     move_tile<float, K, false, NT>(st, tile, sbo, tb, k, lowb, tid);
 }
 
-template <int K, int R>
-__global__ __launch_bounds__(1 << (K - R)) void k_mux(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
+// K = 13: two workgroups per CU (two 64 KiB tiles in LDS), so at most 128 VGPRs
+template <int K, int R, bool DN>
+__global__ __launch_bounds__(1 << (K - R), K == 13 ? 2 : 1) void k_mux(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                                       const GroupDev *__restrict__ gd,
                                                       const OpDev<float> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    mux_tile<K, R>(st, pd, gd, od, wd, tile_order(blockIdx.x, pd), smem);
+    mux_tile<K, R, DN>(st, pd, gd, od, wd, tile_order(blockIdx.x, pd), smem);
 }
 
-template <int K, int R>
+template <int K, int R, bool DN>
 __global__ __launch_bounds__(1 << (K - R)) void k_mux_batch(const BatchItem *__restrict__ items, int n_items) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint64_t b = blockIdx.x;
@@ -438,38 +448,55 @@ __global__ __launch_bounds__(1 << (K - R)) void k_mux_batch(const BatchItem *__r
     };
     const PassDev *pd = uni(it.pd);
     const uint64_t t0 = uni(reinterpret_cast<const char *>(it.tile0)) - static_cast<const char *>(nullptr);
-    mux_tile<K, R>(reinterpret_cast<C2<float> *>(uni(it.st)), pd, uni(it.gd),
+    mux_tile<K, R, DN>(reinterpret_cast<C2<float> *>(uni(it.st)), pd, uni(it.gd),
                    reinterpret_cast<const OpDev<float> *>(uni(it.od)), uni(it.wd), tile_order(b - t0, pd), smem);
 }
 
 // register bits of the k_mux instance for a workgroup tile of 2^K amplitudes
 int mux_register_bits(int K) { return K >= 14 ? 5 : (K == 13 ? kMuxR13 : 4); }
 
-hipError_t launch_mux(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
+#define SVX_MUX_CASES(DN)                                                                                   \
+    switch (K) {                                                                                            \
+    case 10: hipLaunchKernelGGL((k_mux<10, 4, DN>), dim3(grid), dim3(1 << 6), lds, s, p, pd, gd, od, wd); break; \
+    case 11: hipLaunchKernelGGL((k_mux<11, 4, DN>), dim3(grid), dim3(1 << 7), lds, s, p, pd, gd, od, wd); break; \
+    case 12: hipLaunchKernelGGL((k_mux<12, 4, DN>), dim3(grid), dim3(1 << 8), lds, s, p, pd, gd, od, wd); break; \
+    case 13: hipLaunchKernelGGL((k_mux<13, kMuxR13, DN>), dim3(grid), dim3(1 << (13 - kMuxR13)), lds, s, p, pd, gd, od, wd); break; \
+    case 14: hipLaunchKernelGGL((k_mux<14, 5, DN>), dim3(grid), dim3(1 << 9), lds, s, p, pd, gd, od, wd); break; \
+    default: return hipErrorInvalidValue;                                                                   \
+    }
+
+hipError_t launch_mux(int K, bool dense, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
                       const float *wd, uint32_t grid, size_t lds, hipStream_t s) {
     C2<float> *p = reinterpret_cast<C2<float> *>(st);
-    switch (K) {
-    case 10: hipLaunchKernelGGL((k_mux<10, 4>), dim3(grid), dim3(1 << 6), lds, s, p, pd, gd, od, wd); break;
-    case 11: hipLaunchKernelGGL((k_mux<11, 4>), dim3(grid), dim3(1 << 7), lds, s, p, pd, gd, od, wd); break;
-    case 12: hipLaunchKernelGGL((k_mux<12, 4>), dim3(grid), dim3(1 << 8), lds, s, p, pd, gd, od, wd); break;
-    case 13: hipLaunchKernelGGL((k_mux<13, kMuxR13>), dim3(grid), dim3(1 << (13 - kMuxR13)), lds, s, p, pd, gd, od, wd); break;
-    case 14: hipLaunchKernelGGL((k_mux<14, 5>), dim3(grid), dim3(1 << 9), lds, s, p, pd, gd, od, wd); break;
-    default: return hipErrorInvalidValue;
+    if (dense) {
+        SVX_MUX_CASES(true)
+    } else {
+        SVX_MUX_CASES(false)
     }
     return hipGetLastError();
 }
+#undef SVX_MUX_CASES
 
-hipError_t launch_mux_batch(int K, const BatchItem *items, int n_items, uint32_t grid, size_t lds, hipStream_t s) {
-    switch (K) {
-    case 10: hipLaunchKernelGGL((k_mux_batch<10, 4>), dim3(grid), dim3(1 << 6), lds, s, items, n_items); break;
-    case 11: hipLaunchKernelGGL((k_mux_batch<11, 4>), dim3(grid), dim3(1 << 7), lds, s, items, n_items); break;
-    case 12: hipLaunchKernelGGL((k_mux_batch<12, 4>), dim3(grid), dim3(1 << 8), lds, s, items, n_items); break;
-    case 13: hipLaunchKernelGGL((k_mux_batch<13, kMuxR13>), dim3(grid), dim3(1 << (13 - kMuxR13)), lds, s, items, n_items); break;
-    case 14: hipLaunchKernelGGL((k_mux_batch<14, 5>), dim3(grid), dim3(1 << 9), lds, s, items, n_items); break;
-    default: return hipErrorInvalidValue;
+#define SVX_MUXB_CASES(DN)                                                                                   \
+    switch (K) {                                                                                             \
+    case 10: hipLaunchKernelGGL((k_mux_batch<10, 4, DN>), dim3(grid), dim3(1 << 6), lds, s, items, n_items); break; \
+    case 11: hipLaunchKernelGGL((k_mux_batch<11, 4, DN>), dim3(grid), dim3(1 << 7), lds, s, items, n_items); break; \
+    case 12: hipLaunchKernelGGL((k_mux_batch<12, 4, DN>), dim3(grid), dim3(1 << 8), lds, s, items, n_items); break; \
+    case 13: hipLaunchKernelGGL((k_mux_batch<13, kMuxR13, DN>), dim3(grid), dim3(1 << (13 - kMuxR13)), lds, s, items, n_items); break; \
+    case 14: hipLaunchKernelGGL((k_mux_batch<14, 5, DN>), dim3(grid), dim3(1 << 9), lds, s, items, n_items); break; \
+    default: return hipErrorInvalidValue;                                                                    \
+    }
+
+hipError_t launch_mux_batch(int K, bool dense, const BatchItem *items, int n_items, uint32_t grid, size_t lds,
+                            hipStream_t s) {
+    if (dense) {
+        SVX_MUXB_CASES(true)
+    } else {
+        SVX_MUXB_CASES(false)
     }
     return hipGetLastError();
 }
+#undef SVX_MUXB_CASES
 
 hipError_t set_mux_attributes() {
     const int big = 160 * 1024;
@@ -477,9 +504,14 @@ hipError_t set_mux_attributes() {
 #define SVX_ATTR(fn)                                                                              \
     if (e == hipSuccess) e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),               \
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    SVX_ATTR((k_mux<10, 4>)) SVX_ATTR((k_mux<11, 4>)) SVX_ATTR((k_mux<12, 4>)) SVX_ATTR((k_mux<13, kMuxR13>))
-    SVX_ATTR((k_mux<14, 5>)) SVX_ATTR((k_mux_batch<10, 4>)) SVX_ATTR((k_mux_batch<11, 4>))
-    SVX_ATTR((k_mux_batch<12, 4>)) SVX_ATTR((k_mux_batch<13, kMuxR13>)) SVX_ATTR((k_mux_batch<14, 5>))
+#define SVX_ATTR_DN(DN)                                                                                      \
+    SVX_ATTR((k_mux<10, 4, DN>)) SVX_ATTR((k_mux<11, 4, DN>)) SVX_ATTR((k_mux<12, 4, DN>))                  \
+    SVX_ATTR((k_mux<13, kMuxR13, DN>)) SVX_ATTR((k_mux<14, 5, DN>)) SVX_ATTR((k_mux_batch<10, 4, DN>))        \
+    SVX_ATTR((k_mux_batch<11, 4, DN>)) SVX_ATTR((k_mux_batch<12, 4, DN>)) SVX_ATTR((k_mux_batch<13, kMuxR13, DN>)) \
+    SVX_ATTR((k_mux_batch<14, 5, DN>))
+    SVX_ATTR_DN(false)
+    SVX_ATTR_DN(true)
+#undef SVX_ATTR_DN
 #undef SVX_ATTR
     return e;
 }

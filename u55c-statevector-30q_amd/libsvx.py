@@ -23,7 +23,7 @@ SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -
 SYMBOLS = ["svx_init", "svx_run_segment", "svx_run_segment_ex", "svx_engine", "svx_default_options",
            "svx_prepare", "svx_prepare_ex", "svx_release_prepared", "svx_cache_limit", "svx_prepared_shape",
            "svx_run_prepared", "svx_tune_prepared", "svx_prepared_info", "svx_prepared_kernel",
-           "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_swap_words", "svx_zero_state",
+           "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_permute_plan", "svx_swap_words", "svx_zero_state",
            "svx_product_state", "svx_reduce_f64", "svx_reduce_f64_noalloc", "svx_fidelity_f64", "svx_convert", "svx_sync",
            "svx_last_error", "svx_version"]
 
@@ -119,6 +119,7 @@ _SIGS = {
     "svx_prepared_pass_stats": ([_I64, _I32, _PI32, ], _I32),
     "svx_run_batch": ([ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int64), _PI32, _I32, _VP], _I32),
     "svx_permute": ([_VP, _VP, _I32, _PI32, _I32, _VP], _I32),
+    "svx_permute_plan": ([_I32, _PI32, _I32, _PI32, _I32, _PI32], _I32),
     "svx_swap_words": ([_VP, _VP, _I64, _I64, _I64, _VP], _I32),
     "svx_zero_state": ([_VP, _I32, _I32, _VP], _I32),
     "svx_product_state": ([_VP, _I32, _I32, _PD, _VP], _I32),
@@ -332,8 +333,11 @@ def prepared_pass_stats(handle, pass_index):
 
 
 def permute(dst, src, perm, stream=None):
-    """svx_permute: dst = src with perm[new_bit] = old_bit (bit-exact, out of place)."""
+    """svx_permute: dst = src with perm[new_bit] = old_bit (bit-exact).  dst None or dst is src:
+    in place (a few sweeps, no second state)."""
     lib = load()
+    if dst is None:
+        dst = src
     if dst.dtype != src.dtype or dst.numel() != src.numel():
         raise ValueError("dst and src must have the same dtype and size")
     n = _nbits(src)
@@ -341,6 +345,19 @@ def permute(dst, src, perm, stream=None):
     if p.size != n:
         raise ValueError("perm must have one entry per qubit")
     _check(lib.svx_permute(_dev(dst), _dev(src), n, pp, dtype_code(src), _stream(stream)))
+
+
+def permute_plan(perm, in_place=True):
+    """svx_permute_plan: the step permutations of svx_permute (one per HBM sweep), as an int array
+    of shape (steps, n)."""
+    p, pp = _i32(perm)
+    n = p.size
+    cnt = ctypes.c_int32(0)
+    lib = load()
+    _check(lib.svx_permute_plan(n, pp, int(bool(in_place)), None, 0, ctypes.byref(cnt)))
+    out = np.zeros((cnt.value, n), dtype=np.int32)
+    _check(lib.svx_permute_plan(n, pp, int(bool(in_place)), out.ctypes.data_as(_PI32), cnt.value, ctypes.byref(cnt)))
+    return out
 
 
 def swap_words(a, b, offa, offb, nwords, stream=None):

@@ -1,10 +1,17 @@
 """SVRuntime: the reference runtime's interface (host/sv/runtime.py:83-334) on one MI355X.
 
 `run(qc, ...)` compiles, zeroes the state on the device, launches every segment through libsvx
-(the drop-in for the XRT run call), executes remaps as one permutation sweep each (instead of the
-reference's transposition loop over sv_swap / host copies, host/sv/runtime.py:235-284), reads the
-state back, samples shots and returns the reference's dict shape:
-dict(timing, P, L, k, n_run, stats, counts[, state]).
+(the drop-in for the XRT run call), executes remaps on the device (instead of the reference's
+transposition loop over sv_swap / host copies, host/sv/runtime.py:235-284), reads the state back,
+samples shots and returns the reference's dict shape: dict(timing, P, L, k, n_run, stats, counts[, state]).
+
+Remaps run out of place (one sweep into a second state buffer) when a second state fits in device
+memory, otherwise in place (a few sweeps, no second buffer: libsvx.permute with dst = src).  Like the
+reference's device-swap self-test with its host-exchange fallback (host/sv/runtime.py:136-150,
+192-199), the runtime checks both device permutation paths once on a small state and falls back from
+the out-of-place path to the in-place one when its self-test fails or its buffer cannot be allocated;
+each fallback is recorded in `log`.  A state that does not fit raises RuntimeError (the reference's
+capacity error, host/sv/runtime.py:220-221).
 
 target="mi355x" (default) compiles with the MI355X retarget (one slice, 2^13-amplitude LDS tile, up
 to 7 gathered high qubits, DAG-aware pass packing); target="reference" keeps the reference's
@@ -159,6 +166,7 @@ class SVRuntime:
         self.st = None
         self.tmp = None
         self.log = []
+        self.remap_mode = None      # "out_of_place" / "in_place", set by the self-test on first use
 
     # ------------------------------------------------------------------ planning
     def plan(self, n, p_max=None):
@@ -178,25 +186,99 @@ class SVRuntime:
         n_run = max(n, LMIN + P.bit_length() - 1)
         return P, n_run, n_run - (P.bit_length() - 1)
 
-    def admit(self, n, dtype=None):
-        """Footprint admission: the state (2^n amplitudes) plus one permutation buffer must fit."""
+    def _held(self):
+        return sum(t.numel() * t.element_size() for t in (self.st, self.tmp) if t is not None)
+
+    def admit(self, n, dtype=None, buffers=2):
+        """Footprint admission: `buffers` states of 2^n amplitudes (2: the state and an out-of-place
+        remap buffer; 1: the state alone, remaps in place) fit in the free device memory plus what
+        this runtime already holds."""
         dt = self.dtype if dtype is None else dtype
         es = 8 if dt == self.torch.complex64 else 16
-        need = (2 << n) * es
+        need = buffers * (1 << n) * es
         free, total = self.torch.cuda.mem_get_info(self.device)
-        return need <= free + (self.st.numel() * self.st.element_size() * 2 if self.st is not None else 0)
+        return need <= free + self._held()
 
     def _alloc(self, n):
         torch = self.torch
         if self.st is None or self.st.numel() != (1 << n) or self.st.dtype != self.dtype:
+            if not self.admit(n, buffers=1):
+                es = 8 if self.dtype == torch.complex64 else 16
+                raise RuntimeError(f"state of 2^{n} amplitudes ({((1 << n) * es) >> 20} MiB) does not fit "
+                                   "in device memory")
             self.st = None
             self.tmp = None
             self.st = torch.empty(1 << n, dtype=self.dtype, device=f"cuda:{self.device}")
 
     def _tmp(self):
         if self.tmp is None or self.tmp.numel() != self.st.numel() or self.tmp.dtype != self.st.dtype:
+            self.tmp = None
             self.tmp = self.torch.empty_like(self.st)
         return self.tmp
+
+    def self_test(self):
+        """Both device permutation paths on a seeded 14-qubit state against the same permutation of
+        the host copy (a bit reversal: data movement only, so equality is exact).  Sets `remap_mode`:
+        out of place when that path passes, else in place when that one passes; RuntimeError when
+        neither does."""
+        torch = self.torch
+        n = 14
+        rng = np.random.default_rng(14)
+        host = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)).astype(np.complex64)
+        perm = list(range(n))[::-1]
+        want = host.reshape([2] * n).transpose([n - 1 - perm[n - 1 - i] for i in range(n)]).reshape(-1)
+        src = torch.from_numpy(host).to(f"cuda:{self.device}")
+        ok = {}
+        try:
+            dst = torch.empty_like(src)
+            self._permute_oop(dst, src, perm)
+            ok["out_of_place"] = bool(np.array_equal(dst.cpu().numpy(), want))
+        except RuntimeError as e:
+            ok["out_of_place"] = False
+            self.log.append(f"out-of-place permutation self-test raised: {e}")
+        try:
+            self._permute_inplace(src, perm)
+            ok["in_place"] = bool(np.array_equal(src.cpu().numpy(), want))
+        except RuntimeError as e:
+            ok["in_place"] = False
+            self.log.append(f"in-place permutation self-test raised: {e}")
+        if ok["out_of_place"]:
+            self.remap_mode = "out_of_place"
+        elif ok["in_place"]:
+            self.remap_mode = "in_place"
+            self.log.append("out-of-place permutation self-test failed: in-place remaps")
+        else:
+            raise RuntimeError("device permutation self-test failed on both paths")
+        return ok
+
+    # the two device paths (separate methods so a failure can be injected in one of them)
+    def _permute_oop(self, dst, src, perm, stream=None):
+        libsvx.permute(dst, src, perm, stream=stream)
+
+    def _permute_inplace(self, st, perm, stream=None):
+        libsvx.permute(st, st, perm, stream=stream)
+
+    def remap(self, perm, stream=None):
+        """Apply perm[new] = old to the resident state: out of place when a second state fits, else
+        (or after an out-of-place failure) in place."""
+        if self.remap_mode is None:
+            self.self_test()
+        if self.remap_mode == "out_of_place":
+            if self.tmp is None and not self.admit(self.st.numel().bit_length() - 1, self.st.dtype):
+                self.log.append(f"second state of 2^{self.st.numel().bit_length() - 1} amplitudes does not fit: "
+                                "in-place remap")
+            else:
+                try:
+                    tmp = self._tmp()
+                    self._permute_oop(tmp, self.st, perm, stream=stream)
+                    self.st, self.tmp = tmp, self.st
+                    return "out_of_place"
+                except RuntimeError as e:   # includes torch.cuda.OutOfMemoryError
+                    self.tmp = None
+                    self.remap_mode = "in_place"
+                    self.log.append(f"out-of-place remap failed ({e}): in-place remaps from here on")
+        self._permute_inplace(self.st, perm, stream=stream)
+        return "in_place"
 
     def compile(self, qc, k=None, fuse=True, tiling=True, initial_map=None, p_max=None):
         n = qc.num_qubits
@@ -238,19 +320,22 @@ class SVRuntime:
                 torch.cuda.synchronize(self.device)
                 T["T_kernel"] = T.get("T_kernel", 0.0) + time.perf_counter() - a
             else:
-                tmp = self._tmp()
-                libsvx.permute(tmp, self.st, payload, stream=stream)
-                self.st, self.tmp = tmp, self.st
+                self.remap(payload, stream=stream)
                 torch.cuda.synchronize(self.device)
                 T["T_exchange"] = T.get("T_exchange", 0.0) + time.perf_counter() - a
         return T
 
     def logical_state(self, fmap, n_run):
-        """State in logical order (host/sv/runtime.py:327-330), permuted on the device."""
+        """State in logical order (host/sv/runtime.py:327-330), permuted on the device (in place
+        when a second state does not fit: the resident state is then left in logical order)."""
         if list(fmap) == list(range(n_run)):
             return self.st
+        perm = [fmap[l] for l in range(n_run)]
+        if self.remap_mode == "in_place" or (self.tmp is None and not self.admit(n_run, self.st.dtype)):
+            self._permute_inplace(self.st, perm)
+            return self.st
         tmp = self._tmp()
-        libsvx.permute(tmp, self.st, [fmap[l] for l in range(n_run)])
+        self._permute_oop(tmp, self.st, perm)
         return tmp
 
     def run(self, qc, shots=1024, k=None, p_max=None, fuse=True, tiling=True, initial_map=None,

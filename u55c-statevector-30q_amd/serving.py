@@ -11,11 +11,15 @@ Requests carry compiled QSV2 programs (the client compiles; `compile_request`). 
 - runs large circuits (>= `graph_min_qubits`) as a captured HIP graph of their whole pass sequence,
   replayed on a second stream next to the batched steps;
 - reduces the 2^8 marginal probabilities of logical qubits 0..7 in FP64 on a third stream once a
-  circuit's last pass has completed (host/sv/runtime.py:47-80 samples from the same marginals).
+  circuit's last pass has completed (host/sv/runtime.py:47-80 samples from the same marginals); the
+  reduction and its read-back run on a collector thread, so the request loop keeps issuing steps
+  while a large state is being read.
 
 Results do not depend on the mix: a batched pass runs the same device code as a single run
 (svx_run_batch), so each state is bit-identical alone and in the mix.
 """
+import collections
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -91,6 +95,49 @@ class SVServer:
         self.t0 = None
         self.n_steps = 0
         self.n_launch_items = 0
+        # collector thread: FP64 reductions of finished states (one at a time: the library's
+        # reduction scratch is shared), results handed back to the loop through `reduced`
+        self._to_reduce = collections.deque()
+        self.reduced = collections.deque()
+        self._cv = threading.Condition()
+        self._stop = False
+        self._collector = None
+        self.collector_error = None
+        self._handed = 0
+        self._taken = 0
+
+    def _collector_loop(self):
+        torch = self.torch
+        torch.cuda.set_device(self.device)
+        while True:
+            with self._cv:
+                while not self._to_reduce and not self._stop:
+                    self._cv.wait()
+                if not self._to_reduce:
+                    return
+                r = self._to_reduce.popleft()
+            try:
+                self.s_red.wait_event(r.done_event)
+                fm = list(r.fmap[:N_MARGINAL])
+                r.norm, r.marginals = libsvx.reduce_f64(r.state, fm, stream=self.s_red.cuda_stream)
+                r.t_done = self.now()
+            except Exception as e:   # reported by the loop thread
+                self.collector_error = e
+            self.reduced.append(r)
+
+    def _start_collector(self):
+        if self._collector is None:
+            self._stop = False
+            self._collector = threading.Thread(target=self._collector_loop, daemon=True)
+            self._collector.start()
+
+    def _stop_collector(self):
+        if self._collector is not None:
+            with self._cv:
+                self._stop = True
+                self._cv.notify()
+            self._collector.join()
+            self._collector = None
 
     def now(self):
         return time.perf_counter() - self.t0
@@ -186,25 +233,38 @@ class SVServer:
         self.graphs = []
 
     def _collect(self, block=False):
-        torch = self.torch
+        """Hand finished circuits (last pass complete) to the collector thread, and take back the
+        reduced ones: free their states and give back their handles.  `block`: wait for one."""
         left = []
         for r in self.finishing:
             if block:
                 r.done_event.synchronize()
             if r.done_event.query():
-                self.s_red.wait_event(r.done_event)
-                fm = list(r.fmap[:N_MARGINAL])
-                r.norm, r.marginals = libsvx.reduce_f64(r.state, fm, stream=self.s_red.cuda_stream)
-                r.t_done = self.now()
-                if not self.keep_states:
-                    r.state = None
-                    self.in_use -= r.footprint
-                r.graph = None            # replayed once; its handles are given back below
-                self._release(r)
-                self.done.append(r)
+                with self._cv:
+                    self._to_reduce.append(r)
+                    self._handed += 1
+                    self._cv.notify()
             else:
                 left.append(r)
         self.finishing = left
+        if block:
+            while not self.reduced and self.outstanding():
+                time.sleep(0.0002)
+        while self.reduced:
+            r = self.reduced.popleft()
+            self._taken += 1
+            if self.collector_error is not None:
+                raise self.collector_error
+            if not self.keep_states:
+                r.state = None
+                self.in_use -= r.footprint
+            r.graph = None            # replayed once; its handles are given back below
+            self._release(r)
+            self.done.append(r)
+
+    def outstanding(self):
+        """Circuits handed to the collector whose results the loop has not taken back yet."""
+        return self._handed - self._taken
 
     def serve(self, requests, realtime=True):
         """Run a request stream: each request is submitted at its arrival time (seconds from the
@@ -212,8 +272,18 @@ class SVServer:
         Returns the completed requests."""
         pending = sorted(requests, key=lambda r: r.arrival)
         self.t0 = time.perf_counter()
+        self._start_collector()
+        try:
+            self._loop(pending, realtime)
+        finally:
+            self._stop_collector()
+        self.torch.cuda.synchronize(self.device)
+        return self.done
+
+    def _loop(self, pending, realtime):
         i = 0
-        while i < len(pending) or self.queue or self.active or self.graphs or self.finishing:
+        while (i < len(pending) or self.queue or self.active or self.graphs or self.finishing
+               or self.outstanding()):
             t = self.now()
             while i < len(pending) and (not realtime or pending[i].arrival <= t):
                 self.submit(pending[i])
@@ -223,16 +293,14 @@ class SVServer:
                 self._step()
             idle = not (self.active or self.graphs)
             # with nothing to step, wait for a result unless an arrival is due first
-            self._collect(block=idle and bool(self.finishing) and i >= len(pending))
+            self._collect(block=idle and bool(self.finishing or self.outstanding()) and i >= len(pending))
             if idle and i < len(pending):
                 wait = pending[i].arrival - self.now()
                 if wait > 0:
-                    time.sleep(min(wait, 0.0005 if self.finishing else 0.002))
-            if self.queue and not (self.active or self.graphs or self.finishing):
+                    time.sleep(min(wait, 0.0005 if (self.finishing or self.outstanding()) else 0.002))
+            if self.queue and not (self.active or self.graphs or self.finishing or self.outstanding()):
                 if self.in_use + self.queue[0].footprint > self.budget:
                     raise RuntimeError("a queued request can never be admitted")
-        self.torch.cuda.synchronize(self.device)
-        return self.done
 
 
 # ---------------------------------------------------------------------- request streams
