@@ -60,7 +60,8 @@ typedef struct svx_options {
     int32_t dense_math;      /* SVX_DENSE_* */
     int32_t kernel;          /* SVX_KERNEL_* */
     int32_t diag_tables;     /* 1 (default): runs of diagonal ops as one table of phases; 0: never */
-    int32_t dense_only;      /* 1 (default): passes of 4-qubit units only on the 512-thread kernel; 0: never */
+    int32_t dense_only;      /* 1 (default): split-FP16 passes of 4-qubit units only on the 512-thread
+                                dense-only kernel (FP32 passes run on the MUX2 kernel); 0: never */
     int32_t tile_scale;      /* split-FP16 only: 1 (default) per-tile pre-scaling, 0 per-column scaling */
     int32_t tile_order;      /* SVX_ORDER_* */
     int32_t mux_engine;      /* SVX_MUX_* */
@@ -165,6 +166,23 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
  * pass (-2: each XCD takes a contiguous range of tiles, 0: identity; see svx_tune_prepared)
  * (out has 8 entries).  No reference counterpart. */
 int svx_prepared_pass_stats(int64_t handle, int32_t pass, int32_t *out);
+
+/* Persistent queue of whole-state complex64 circuits of 2^state_bits amplitudes (state_bits 10..13):
+ * a resident kernel (`workgroups` workgroups on a stream of the library's own) takes the jobs from a
+ * host-visible ring of `capacity` slots, each workgroup one circuit at a time with the state in LDS,
+ * so many circuits run in one launch with no launch per circuit.  A job is a prepared segment of one
+ * whole-state pass on the MUX2 kernel (compiled with k = state_bits; FP32 dense units included) and a
+ * state it runs on (zero_init: the state starts as |0...0>, nothing is read); the handle must stay
+ * referenced until the job is done.  Results are bit-identical to svx_run_prepared on the same state
+ * (same device code).  The kernel leaves after idle_us microseconds without a job and is relaunched
+ * by submit / done / wait when needed; close stops it.  svx_queue_submit returns SVX_E_CAPACITY when
+ * every slot holds an unfinished job.  No reference counterpart (the reference's latency study runs
+ * one circuit per call, bench/latency_jitter.py:24-58). */
+int svx_queue_open(int32_t state_bits, int32_t capacity, int32_t workgroups, int32_t idle_us, int64_t *queue);
+int svx_queue_submit(int64_t queue, void *state, int64_t handle, int32_t zero_init, int64_t *ticket);
+int svx_queue_done(int64_t queue, int64_t ticket, int32_t *done);
+int svx_queue_wait(int64_t queue, int64_t ticket);
+int svx_queue_close(int64_t queue);
 
 /* Qubit permutation perm[new_bit] = old_bit of a 2^n state, bit-exact.
  * Replaces the reference's remap paths: model.remap (host/sv/model.py:74-78), the host transpose

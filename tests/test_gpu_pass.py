@@ -345,9 +345,10 @@ def test_pass_fusion_double_buffered_kernel(gpu, n, k):
 @pytest.mark.parametrize("math", ["f32", "split16"])
 @pytest.mark.parametrize("n,k", [(18, 12), (20, 13), (22, 13)])
 def test_dense_only_kernel_matches_general_kernel(gpu, n, k, math):
-    """Every op of every pass in a dense 4-qubit unit (fuse=dense4) runs on the 512-thread
-    dense-only kernel; it is bit-identical to the general dense kernel (dense_only off: same units,
-    same arithmetic, other thread count) and matches the complex128 golden model."""
+    """Every op of every pass in a dense 4-qubit unit (fuse=dense4).  Split FP16: the passes run on
+    the 512-thread dense-only kernel, bit-identical to the general dense kernel (dense_only off: same
+    units, same arithmetic, other thread count).  FP32: the MUX2 kernel runs them either way.  Both
+    match the complex128 golden model."""
     from u55c_statevector_30q_amd import libsvx
     rng = np.random.default_rng(3 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)
@@ -364,8 +365,11 @@ def test_dense_only_kernel_matches_general_kernel(gpu, n, k, math):
             _check_c64(ref, got[d4], (n, k, d4), seg)
         else:
             assert judge.metrics(ref, got[d4])["one_minus_f"] < 1e-9
-    assert any(nm.startswith("k_pass_d4<") for nm in names[True]), names
-    assert not any(nm.startswith("k_pass_d4<") for nm in names[False]), names
+    if math == "f32":
+        assert all(nm.startswith("k_mux<") for nm in names[True] + names[False]), names
+    else:
+        assert any(nm.startswith("k_pass_d4<") for nm in names[True]), names
+        assert not any(nm.startswith("k_pass_d4<") for nm in names[False]), names
     assert np.array_equal(got[True], got[False]), names
 
 
@@ -406,8 +410,9 @@ def test_dense_units_on_chosen_bits(gpu, units):
 @pytest.mark.parametrize("math", ["f32", "split16_tile", "split16_col"])
 @pytest.mark.parametrize("n,k", [(24, 13), (23, 12)])
 def test_dense_only_kernel_large_states(gpu, n, k, math):
-    """The dense-only kernel on states of 2^11 tiles (FP32, and split FP16 with pre-scaled tiles and
-    with per-column scaling): bit-identical to the general dense kernel, norm kept."""
+    """Dense-only passes on states of 2^11 tiles: split FP16 (pre-scaled tiles and per-column
+    scaling) on the dense-only kernel, bit-identical to the general dense kernel; FP32 on the MUX2
+    kernel either way; norm kept."""
     from u55c_statevector_30q_amd import libsvx
     rng = np.random.default_rng(5 * n + k)
     seg = random_segment(rng, n, k, n_passes=3, max_ops=30, max_m=6, ctrl_p=0.08)
@@ -422,8 +427,11 @@ def test_dense_only_kernel_large_states(gpu, n, k, math):
             names[d4] = [libsvx.prepared_kernel(h, p) for p in range(libsvx.prepared_info(h)[1])]
             libsvx.release_prepared(h)
             got[d4] = run_hip(s0, seg)
-    assert any(nm.startswith("k_pass_d4<") for nm in names[True]), names
-    assert not any(nm.startswith("k_pass_d4<") for nm in names[False]), names
+    if math == "f32":
+        assert all(nm.startswith("k_mux<") for nm in names[True] + names[False]), names
+    else:
+        assert any(nm.startswith("k_pass_d4<") for nm in names[True]), names
+        assert not any(nm.startswith("k_pass_d4<") for nm in names[False]), names
     assert np.array_equal(got[True], got[False])
     assert abs(np.linalg.norm(got[True].astype(np.complex128)) - 1) < 1e-5
 

@@ -1,5 +1,5 @@
-"""Served mix: continuous batching (svx_run_batch), HIP-graph replay of large circuits and
-admission by footprint.  Each result must be bit-identical alone and in the mix (the batched
+"""Served mix: continuous batching (svx_run_batch), the persistent queue kernel for whole-state
+circuits (svx_queue_*), HIP-graph replay of large circuits and admission by footprint.  Each result must be bit-identical alone and in the mix (the batched
 launch runs the same device code per state), and match the complex128 golden model within the
 complex64 judge tolerances."""
 import numpy as np
@@ -48,6 +48,7 @@ def test_alone_vs_mix_bit_identical(gpu, realtime):
     srv = serving.SVServer(keep_states=True, graph_min_qubits=64)
     done = srv.serve(reqs, realtime=realtime)
     assert len(done) == len(SPEC) and srv.n_steps >= 1
+    assert srv.n_queued == sum(1 for n, _ in SPEC if n <= serving.WHOLE_STATE_MAX)   # the persistent queue kernel
     if not realtime:
         assert srv.n_launch_items > srv.n_steps      # several circuits shared launches
     for r in done:
@@ -102,7 +103,8 @@ def test_graph_replay_matches_eager(gpu):
     done = {r.rid: r for r in srv.serve(reqs, realtime=False)}
     assert done[0].mode == "graph" and bool((done[0].state == eager).all())
     for rid in (1, 2, 3):
-        assert done[rid].mode == "batch" and bool((done[rid].state == _alone(done[rid])).all())
+        mode = "queue" if done[rid].n <= serving.WHOLE_STATE_MAX else "batch"
+        assert done[rid].mode == mode and bool((done[rid].state == _alone(done[rid])).all())
 
 
 def test_admission_by_footprint(gpu):

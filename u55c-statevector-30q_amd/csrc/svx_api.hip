@@ -22,6 +22,8 @@
 #include <memory>
 #include <functional>
 #include <mutex>
+#include <thread>
+#include <chrono>
 #include <set>
 #include <string>
 #include <vector>
@@ -797,21 +799,27 @@ int mux_groups(const std::vector<OpDev<T>> &pops, const std::vector<int> &dense_
         const bool cphase = o.kind == K_CPHASE || o.kind == K_CPHASE1;
         tc[i] = cphase ? sup[i] : (1u << o.t);
     }
-    std::vector<uint64_t> pred(n, 0);
+    // dependency DAG (any number of ops: a whole-state pass may hold hundreds): op j waits for each
+    // earlier op it does not commute with
+    std::vector<std::vector<int>> succ(n);
+    std::vector<int> waiting(n, 0);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < j; ++i)
-            if ((mix[i] & sup[j]) || (mix[j] & sup[i])) pred[j] |= uint64_t(1) << i;
-    uint64_t done = 0;
-    const uint64_t all = n == 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
+            if ((mix[i] & sup[j]) || (mix[j] & sup[i])) {
+                succ[i].push_back(j);
+                ++waiting[j];
+            }
+    std::vector<char> placed(n, 0);
+    int n_placed = 0;
     int n_full = 0;
-    while (done != all) {
+    while (n_placed < n) {
         uint32_t S = 0;
         int nS = 0;
         std::vector<std::pair<int, int>> mem;   // (op, register target tile bit; -1 for a table)
         while (true) {
             int best = -1, bestcost = 99, bestbit = -1;
             for (int i = 0; i < n; ++i) {
-                if (((done >> i) & 1u) || (pred[i] & ~done)) continue;
+                if (placed[i] || waiting[i]) continue;
                 int cost, bit;
                 if (tc[i] == 0) {
                     cost = 0;
@@ -833,7 +841,9 @@ int mux_groups(const std::vector<OpDev<T>> &pops, const std::vector<int> &dense_
                 }
             }
             if (best < 0) break;
-            done |= uint64_t(1) << best;
+            placed[best] = 1;
+            ++n_placed;
+            for (int j : succ[best]) --waiting[j];
             if (bestbit >= 0 && !((S >> bestbit) & 1u)) {
                 S |= 1u << bestbit;
                 ++nS;
@@ -1422,8 +1432,11 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
         }
         pd.ngroups = (int)groups.size() - pd.group0;
         LaunchInfo &lr = launches.back();
-        // dense-only passes of 4-qubit units run on the 512-thread kernel (dense_only = 0: never)
-        if (mode == 1 && has_dense && opt.dense_only && (K == 12 || K == 13) && pd.ngroups > 0 && k - rp.m >= 1) {
+        // dense-only passes of 4-qubit units run on the 512-thread dense-only kernel (dense_only = 0:
+        // never); with FP32 units the MUX2 kernel runs them (measured slightly faster: 194.7 against
+        // 197.1 ms for the 30-qubit workload), so only split-FP16 passes take it
+        if (mode == 1 && has_dense && opt.dense_only && !use_mux && (K == 12 || K == 13) && pd.ngroups > 0 &&
+            k - rp.m >= 1) {
             bool all4 = true;
             for (int g = pd.group0; g < (int)groups.size(); ++g) all4 = all4 && groups[g].type != 0 && groups[g].type != 6 && groups[g].f == 4;
             lr.d4 = all4;
@@ -2226,7 +2239,8 @@ static int perm_entry(DeviceState *ds, int n, const int32_t *perm, const std::ve
 
 static int perm_launch(int dtype, void *dst, const void *src, int n, PermEntry *pe, hipStream_t s) {
     const int v = n <= 12 ? n : 12;
-    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 4)) * (dtype == SVX_C64 ? 8 : 16);
+    // the tile (2 padding elements per 32) and the descriptor's four 64-entry tables
+    const size_t lds = ((size_t(1) << v) + ((size_t(1) << v) >> 4)) * (dtype == SVX_C64 ? 8 : 16) + 64 * (8 + 8 + 2 + 2);
     hipError_t e = launch_permute(dtype, dst, src, (const PermDev *)pe->dev, pe->grid, lds, s);
     if (e != hipSuccess) return hipfail(e, "permute launch");
     if (!pe->pinned) (void)hipEventRecord(pe->ev, s);
@@ -2490,6 +2504,157 @@ int svx_fidelity_f64(const void *a, int32_t dta, const void *b, int32_t dtb, int
 
 int svx_sync(void *stream) {
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------ queue
+// Persistent queue of whole-state circuits (k_mux_queue).  Jobs and completion words live in
+// host-visible coherent memory; the claim counter in device memory.  One resident launch serves
+// every job published while it runs; it drains by itself when idle, and submit / done / wait
+// relaunch it when jobs are pending and the previous launch has finished.
+struct SmallQueue {
+    int device = 0, K = 0;
+    uint32_t cap = 0, wgs = 0;
+    size_t lds = 0;
+    uint64_t idle_ticks = 0;
+    hipStream_t stream = nullptr;
+    QueueJob *jobs = nullptr;
+    QueueCtl *ctl = nullptr;
+    uint32_t *next = nullptr;
+    uint64_t tickets = 0;
+    bool launched = false;
+    std::mutex mu;
+};
+std::mutex g_queue_mu;
+std::vector<std::unique_ptr<SmallQueue>> g_queues;
+
+static int get_queue(int64_t q, SmallQueue **out) {
+    std::lock_guard<std::mutex> lk(g_queue_mu);
+    if (q < 1 || q > (int64_t)g_queues.size() || !g_queues[q - 1]) return fail(SVX_E_ARG, "unknown or closed queue");
+    *out = g_queues[q - 1].get();
+    return 0;
+}
+
+// (Re)launch unless a launch is resident or every job is done (caller holds q->mu).  Jobs are
+// claimed in ticket order and a finished launch completed every job it claimed, so after a finished
+// launch the newest job is done exactly when all are.
+static int queue_kick(SmallQueue *q) {
+    if (q->tickets == 0) return 0;
+    if (q->launched) {
+        const hipError_t st = hipStreamQuery(q->stream);
+        if (st == hipErrorNotReady) return 0;   // still resident
+        if (st != hipSuccess) return hipfail(st, "queue kernel");
+        const uint64_t last = q->tickets - 1;
+        if (__atomic_load_n(&q->ctl->done[last % q->cap], __ATOMIC_ACQUIRE) == (uint32_t)(last + 1)) return 0;
+    }
+    const hipError_t e = launch_mux_queue(q->K, q->jobs, q->ctl, q->next, q->cap, q->idle_ticks, q->wgs, q->lds, q->stream);
+    if (e != hipSuccess) return hipfail(e, "queue kernel launch");
+    q->launched = true;
+    return 0;
+}
+
+int svx_queue_open(int32_t state_bits, int32_t capacity, int32_t workgroups, int32_t idle_us, int64_t *queue) {
+    if (!queue || state_bits < 10 || state_bits > 13 || capacity < 1 || capacity > (1 << 20) || workgroups < 1 ||
+        workgroups > 4096 || idle_us < 1)
+        return fail(SVX_E_ARG, "bad queue arguments (state_bits 10..13, capacity 1..2^20, workgroups 1..4096)");
+    DeviceState *ds = nullptr;
+    int rc = get_ds(&ds);
+    if (rc) return rc;
+    auto q = std::make_unique<SmallQueue>();
+    q->device = ds->caps.device;
+    q->K = state_bits;
+    q->cap = (uint32_t)capacity;
+    q->wgs = (uint32_t)workgroups;
+    q->idle_ticks = uint64_t(idle_us) * 100;   // 100 MHz real-time counter
+    const size_t ne = size_t(1) << state_bits;
+    q->lds = (ne + (ne >> 5)) * 8 + 128 * 8 + 64;
+    HIPCHK(hipHostMalloc((void **)&q->jobs, sizeof(QueueJob) * q->cap, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostMalloc((void **)&q->ctl, sizeof(QueueCtl) + sizeof(uint32_t) * q->cap,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(q->jobs, 0, sizeof(QueueJob) * q->cap);
+    std::memset(q->ctl, 0, sizeof(QueueCtl) + sizeof(uint32_t) * q->cap);
+    HIPCHK(hipMalloc((void **)&q->next, sizeof(uint32_t)));
+    HIPCHK(hipMemset(q->next, 0, sizeof(uint32_t)));
+    HIPCHK(hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking));
+    std::lock_guard<std::mutex> lk(g_queue_mu);
+    g_queues.push_back(std::move(q));
+    *queue = (int64_t)g_queues.size();
+    return 0;
+}
+
+int svx_queue_submit(int64_t queue, void *state, int64_t handle, int32_t zero_init, int64_t *ticket) {
+    SmallQueue *q = nullptr;
+    int rc = get_queue(queue, &q);
+    if (rc) return rc;
+    if (!state || (reinterpret_cast<uintptr_t>(state) & 15)) return fail(SVX_E_ARG, "state must be non-null and 16-byte aligned");
+    Prepared *pr = nullptr;
+    rc = get_prepared(handle, &pr);
+    if (rc) return rc;
+    if (pr->dtype != SVX_C64 || pr->nbits != q->K || pr->device != q->device)
+        return fail(SVX_E_ARG, "queue jobs are complex64 states of the queue's size on its device");
+    if (pr->launches.size() != 1 || !pr->launches[0].mux || pr->launches[0].ntiles != 1)
+        return fail(SVX_E_ARG, "queue jobs are one whole-state pass on the MUX2 kernel (compile with k = state bits)");
+    const LaunchInfo &li = pr->launches[0];
+    std::lock_guard<std::mutex> lk(q->mu);
+    const uint64_t t = q->tickets;
+    const uint32_t slot = (uint32_t)(t % q->cap);
+    if (t >= q->cap && __atomic_load_n(&q->ctl->done[slot], __ATOMIC_ACQUIRE) != (uint32_t)(t - q->cap + 1))
+        return fail(SVX_E_CAPACITY, "queue full: every slot holds a job that is not done");
+    unsigned char *base = static_cast<unsigned char *>(pr->dev);
+    QueueJob &jb = q->jobs[slot];
+    jb.st = reinterpret_cast<uint64_t>(state);
+    jb.pd = reinterpret_cast<uint64_t>(reinterpret_cast<const PassDev *>(base + pr->pass_off) + li.pass_index);
+    jb.gd = reinterpret_cast<uint64_t>(base + pr->group_off);
+    jb.od = reinterpret_cast<uint64_t>(base + pr->op_off);
+    jb.wd = reinterpret_cast<uint64_t>(base + pr->w_off);
+    jb.flags = zero_init ? 1 : 0;
+    q->tickets = t + 1;
+    __atomic_store_n(&q->ctl->tail, (uint32_t)(t + 1), __ATOMIC_RELEASE);
+    rc = queue_kick(q);
+    if (rc) return rc;
+    *ticket = (int64_t)t;
+    return 0;
+}
+
+int svx_queue_done(int64_t queue, int64_t ticket, int32_t *done) {
+    SmallQueue *q = nullptr;
+    int rc = get_queue(queue, &q);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(q->mu);
+    if (ticket < 0 || (uint64_t)ticket >= q->tickets) return fail(SVX_E_ARG, "unknown ticket");
+    const uint32_t w = __atomic_load_n(&q->ctl->done[ticket % q->cap], __ATOMIC_ACQUIRE);
+    if ((uint64_t)ticket + q->cap < q->tickets) {   // its slot was reused: done long ago
+        *done = 1;
+        return 0;
+    }
+    *done = w == (uint32_t)(ticket + 1);
+    if (!*done) return queue_kick(q);
+    return 0;
+}
+
+int svx_queue_wait(int64_t queue, int64_t ticket) {
+    for (int64_t spin = 0;; ++spin) {
+        int32_t d = 0;
+        const int rc = svx_queue_done(queue, ticket, &d);
+        if (rc || d) return rc;
+        if (spin > 4000000) return fail(SVX_E_HIP, "queue job did not complete");
+        std::this_thread::sleep_for(std::chrono::microseconds(spin < 1000 ? 1 : 20));
+    }
+}
+
+int svx_queue_close(int64_t queue) {
+    std::unique_ptr<SmallQueue> q;
+    {
+        std::lock_guard<std::mutex> lk(g_queue_mu);
+        if (queue < 1 || queue > (int64_t)g_queues.size() || !g_queues[queue - 1]) return fail(SVX_E_ARG, "unknown or closed queue");
+        q = std::move(g_queues[queue - 1]);
+    }
+    __atomic_store_n(&q->ctl->stop, 1u, __ATOMIC_RELEASE);
+    HIPCHK(hipStreamSynchronize(q->stream));
+    HIPCHK(hipStreamDestroy(q->stream));
+    HIPCHK(hipFree(q->next));
+    HIPCHK(hipHostFree(q->jobs));
+    HIPCHK(hipHostFree(q->ctl));
     return 0;
 }
 

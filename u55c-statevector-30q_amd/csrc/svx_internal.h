@@ -108,6 +108,18 @@ struct BatchItem {
     uint64_t tile0;              // first workgroup of this item in the launch
 };
 
+// persistent queue of whole-state circuits (k_mux_queue): records in host-visible memory
+struct QueueJob {
+    uint64_t st, pd, gd, od, wd;   // state, and the prepared pass's PassDev / GroupDev / OpDev / operands
+    uint64_t flags;                // bit 0: start from |0...0>
+};
+struct QueueCtl {
+    uint32_t tail;                 // jobs published (host: release store)
+    uint32_t stop;                 // host asks every workgroup to leave
+    uint32_t pad[14];
+    uint32_t done[1];              // [capacity]: ticket + 1 once job `ticket` is complete (device: release)
+};
+
 // launchers (svx_kernels.hip)
 hipError_t launch_pass_batch(int K, bool dense, bool d4, const BatchItem *items, int n_items, uint32_t grid,
                              size_t lds, hipStream_t s);
@@ -130,6 +142,8 @@ hipError_t launch_mux(int K, bool dense, void *st, const PassDev *pd, const Grou
                       const float *wd, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_mux_batch(int K, bool dense, const BatchItem *items, int n_items, uint32_t grid, size_t lds,
                             hipStream_t s);
+hipError_t launch_mux_queue(int K, const QueueJob *jobs, QueueCtl *ctl, uint32_t *next, uint32_t cap,
+                            uint64_t idle_ticks, uint32_t grid, size_t lds, hipStream_t s);
 // double-buffered complex64 pass kernel (one QSV2 tile of 2^K per LDS buffer, K in {12, 13});
 // register groups of 4 tile bits
 size_t pass_dma_lds_bytes(int K, int m);

@@ -1192,8 +1192,21 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_permute(C2<T> *__restrict__ dst, const C2<T> *__restrict__ src,
                                                  const PermDev *__restrict__ pd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    C2<T> *tile = reinterpret_cast<C2<T> *>(smem);
     const int c = pd->c, v = pd->v, nfix = pd->nfix;
+    const uint32_t NE = 1u << v, cm = (1u << c) - 1u;
+    // the descriptor's per-element tables go to LDS first (after the tile), so the address of each
+    // HBM access is two LDS reads away instead of a dependent global load
+    C2<T> *tile = reinterpret_cast<C2<T> *>(smem);
+    uint64_t *t_oldhi = reinterpret_cast<uint64_t *>(smem + (size_t(NE) + (NE >> 4)) * sizeof(C2<T>));
+    uint64_t *t_newhi = t_oldhi + 64;
+    uint16_t *t_rdlo = reinterpret_cast<uint16_t *>(t_newhi + 64);
+    uint16_t *t_rdhi = t_rdlo + 64;
+    if (threadIdx.x < 64) {
+        t_oldhi[threadIdx.x] = pd->oldhi[threadIdx.x];
+        t_newhi[threadIdx.x] = pd->newhi[threadIdx.x];
+        t_rdlo[threadIdx.x] = pd->rdlo[threadIdx.x];
+        t_rdhi[threadIdx.x] = pd->rdhi[threadIdx.x];
+    }
     uint64_t nb = 0, ob = 0;
     const uint64_t wg = blockIdx.x;
     for (int i = 0; i < nfix; ++i) {
@@ -1202,25 +1215,25 @@ __global__ __launch_bounds__(256) void k_permute(C2<T> *__restrict__ dst, const 
             ob |= uint64_t(1) << pd->fixold[i];
         }
     }
-    const uint32_t NE = 1u << v, cm = (1u << c) - 1u;
+    __syncthreads();
     if (sizeof(T) == 4 && c >= 1) {
         // complex64: adjacent element pairs (16 bytes) on both sides; LDS pads 2 elements per 32 so
         // new-order pairs stay 16-byte aligned
         // every 16-byte load of the tile (at most 8 per thread: v <= 12) is issued before the first
         // LDS write, and both sides stream past the caches (each amplitude is moved once)
         auto lp = [](uint32_t e) { return e + ((e >> 5) << 1); };
-        const uint32_t b0 = uint32_t(pd->rdlo[1]);    // LDS position bit of old bit 0
+        const uint32_t b0 = uint32_t(t_rdlo[1]);    // LDS position bit of old bit 0
         float4 r[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const uint32_t q = 2 * threadIdx.x + 512u * i;
-            if (q < NE) r[i] = ld_stream(reinterpret_cast<const float *>(src) + 2 * (ob | (q & cm) | pd->oldhi[q >> c]));
+            if (q < NE) r[i] = ld_stream(reinterpret_cast<const float *>(src) + 2 * (ob | (q & cm) | t_oldhi[q >> c]));
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const uint32_t q = 2 * threadIdx.x + 512u * i;
             if (q < NE) {
-                const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
+                const uint32_t pos = uint32_t(t_rdlo[q & 63u]) | uint32_t(t_rdhi[q >> 6]);
                 tile[lp(pos)] = C2<T>{T(r[i].x), T(r[i].y)};
                 tile[lp(pos | b0)] = C2<T>{T(r[i].z), T(r[i].w)};
             }
@@ -1231,18 +1244,18 @@ __global__ __launch_bounds__(256) void k_permute(C2<T> *__restrict__ dst, const 
             const uint32_t l = 2 * threadIdx.x + 512u * i;
             if (l < NE) {
                 const C2<T> a0 = tile[lp(l)], a1 = tile[lp(l) + 1];
-                st_stream(reinterpret_cast<float *>(dst) + 2 * (nb | (l & cm) | pd->newhi[l >> c]),
+                st_stream(reinterpret_cast<float *>(dst) + 2 * (nb | (l & cm) | t_newhi[l >> c]),
                           make_float4(float(a0.x), float(a0.y), float(a1.x), float(a1.y)));
             }
         }
         return;
     }
     for (uint32_t q = threadIdx.x; q < NE; q += 256) {
-        const uint32_t pos = uint32_t(pd->rdlo[q & 63u]) | uint32_t(pd->rdhi[q >> 6]);
-        tile[lpad(pos)] = src[ob | (q & cm) | pd->oldhi[q >> c]];
+        const uint32_t pos = uint32_t(t_rdlo[q & 63u]) | uint32_t(t_rdhi[q >> 6]);
+        tile[lpad(pos)] = src[ob | (q & cm) | t_oldhi[q >> c]];
     }
     __syncthreads();
-    for (uint32_t l = threadIdx.x; l < NE; l += 256) dst[nb | (l & cm) | pd->newhi[l >> c]] = tile[lpad(l)];
+    for (uint32_t l = threadIdx.x; l < NE; l += 256) dst[nb | (l & cm) | t_newhi[l >> c]] = tile[lpad(l)];
 }
 
 hipError_t launch_permute(int dtype, void *dst, const void *src, const PermDev *pd, uint32_t grid,

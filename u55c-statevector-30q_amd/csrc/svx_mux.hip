@@ -334,10 +334,12 @@ __device__ __forceinline__ void apply(const Rec &rc, const float (&x)[2 << R], f
 // One workgroup tile of a MUX2 pass (k_mux and its batched form): NT = 2^(K-R) threads, one slot each.
 // DN: the pass also holds FP32 dense units (group types 1 and 5), applied by all NT / 64 wavefronts
 // on the padded LDS tile between the register groups.
+// zero: the tile is the whole state and starts as |0...0> (written to LDS, nothing read from HBM).
 template <int K, int R, bool DN>
 __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                          const GroupDev *__restrict__ gd, const OpDev<float> *__restrict__ od,
-                                         const float *__restrict__ wd, uint64_t tile_id, unsigned char *smem) {
+                                         const float *__restrict__ wd, uint64_t tile_id, unsigned char *smem,
+                                         bool zero = false) {
     constexpr int NT = 1 << (K - R);
     constexpr uint32_t NE = 1u << K;
     constexpr uint32_t NEP = NE + (NE >> 5);
@@ -361,7 +363,11 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
     }
     __syncthreads();
     const int lowb = k - m;
-    move_tile<float, K, true, NT>(st, tile, sbo, tb, k, lowb, tid);
+    if (zero) {
+        for (uint32_t e = tid; e < NE; e += NT) tile[lpad(e)] = C2<float>{e == 0 ? 1.0f : 0.0f, 0.0f};
+    } else {
+        move_tile<float, K, true, NT>(st, tile, sbo, tb, k, lowb, tid);
+    }
     __syncthreads();
     const int ng = pd->ngroups, g0 = pd->group0;
     for (int g = 0; g < ng; ++g) {
@@ -452,6 +458,82 @@ __global__ __launch_bounds__(1 << (K - R)) void k_mux_batch(const BatchItem *__r
                    reinterpret_cast<const OpDev<float> *>(uni(it.od)), uni(it.wd), tile_order(b - t0, pd), smem);
 }
 
+// Persistent queue of whole-state circuits (state = one 2^K tile, one pass each).  The host
+// publishes jobs in host-visible memory (QueueJob records, then QueueCtl::tail with a release
+// store); each workgroup claims the next published job with a compare-and-swap on a device counter
+// (never an unpublished one), runs it with the same device code as k_mux, makes its stores visible
+// at device scope, and sets the job's done word (ticket + 1) with a system-scope release store.  A
+// workgroup leaves when the host sets QueueCtl::stop, when no job has arrived for idle_ticks of
+// the 100 MHz real-time counter, or after 2^24 idle rounds, so every launch drains by itself; the
+// host relaunches on demand.
+template <int K, int R>
+__global__ __launch_bounds__(1 << (K - R)) void k_mux_queue(const QueueJob *jobs, QueueCtl *ctl, uint32_t *next,
+                                                            uint32_t cap, uint64_t idle_ticks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint64_t s_job[7];
+    constexpr uint32_t kNone = 0xffffffffu;
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint32_t claimed = kNone;
+            uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            // each idle round reads host memory and sleeps; the round cap bounds the wait even if the
+            // clock does not advance
+            for (uint32_t round = 0; round < (1u << 24); ++round) {
+                const uint32_t tail = __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                uint32_t cur = __hip_atomic_load(next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cur < tail) {
+                    if (__hip_atomic_compare_exchange_strong(next, &cur, cur + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        claimed = cur;
+                        break;
+                    }
+                    continue;
+                }
+                if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+                if (__builtin_amdgcn_s_memrealtime() - t0 > idle_ticks) break;
+                __builtin_amdgcn_s_sleep(8);
+            }
+            s_job[0] = claimed;
+            if (claimed != kNone) {
+                const QueueJob &jb = jobs[claimed % cap];
+                s_job[1] = __hip_atomic_load(&jb.st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                s_job[2] = __hip_atomic_load(&jb.pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                s_job[3] = __hip_atomic_load(&jb.gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                s_job[4] = __hip_atomic_load(&jb.od, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                s_job[5] = __hip_atomic_load(&jb.wd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                s_job[6] = __hip_atomic_load(&jb.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        __syncthreads();
+        const uint32_t j = (uint32_t)s_job[0];
+        if (j == kNone) return;   // the whole workgroup leaves together
+        const uint64_t flags = s_job[6];
+        auto *st = reinterpret_cast<C2<float> *>(s_job[1]);
+        const auto *pd = reinterpret_cast<const PassDev *>(s_job[2]);
+        const auto *gd = reinterpret_cast<const GroupDev *>(s_job[3]);
+        const auto *od = reinterpret_cast<const OpDev<float> *>(s_job[4]);
+        const auto *wd = reinterpret_cast<const float *>(s_job[5]);
+        mux_tile<K, R, true>(st, pd, gd, od, wd, 0, smem, (flags & 1u) != 0);
+        __threadfence();   // this thread's state stores reach device scope
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&ctl->done[j % cap], j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();   // s_job is rewritten by the next claim
+    }
+}
+
+hipError_t launch_mux_queue(int K, const QueueJob *jobs, QueueCtl *ctl, uint32_t *next, uint32_t cap,
+                            uint64_t idle_ticks, uint32_t grid, size_t lds, hipStream_t s) {
+    switch (K) {
+    case 10: hipLaunchKernelGGL((k_mux_queue<10, 4>), dim3(grid), dim3(1 << 6), lds, s, jobs, ctl, next, cap, idle_ticks); break;
+    case 11: hipLaunchKernelGGL((k_mux_queue<11, 4>), dim3(grid), dim3(1 << 7), lds, s, jobs, ctl, next, cap, idle_ticks); break;
+    case 12: hipLaunchKernelGGL((k_mux_queue<12, 4>), dim3(grid), dim3(1 << 8), lds, s, jobs, ctl, next, cap, idle_ticks); break;
+    case 13: hipLaunchKernelGGL((k_mux_queue<13, kMuxR13>), dim3(grid), dim3(1 << (13 - kMuxR13)), lds, s, jobs, ctl, next, cap, idle_ticks); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // register bits of the k_mux instance for a workgroup tile of 2^K amplitudes
 int mux_register_bits(int K) { return K >= 14 ? 5 : (K == 13 ? kMuxR13 : 4); }
 
@@ -511,6 +593,13 @@ hipError_t set_mux_attributes() {
     SVX_ATTR((k_mux_batch<14, 5, DN>))
     SVX_ATTR_DN(false)
     SVX_ATTR_DN(true)
+    // the queue kernel also holds a little static LDS (its job record)
+#define SVX_ATTRQ(fn)                                                                             \
+    if (e == hipSuccess) e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),               \
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, big - 1024);
+    SVX_ATTRQ((k_mux_queue<10, 4>)) SVX_ATTRQ((k_mux_queue<11, 4>)) SVX_ATTRQ((k_mux_queue<12, 4>))
+    SVX_ATTRQ((k_mux_queue<13, kMuxR13>))
+#undef SVX_ATTRQ
 #undef SVX_ATTR_DN
 #undef SVX_ATTR
     return e;

@@ -23,7 +23,8 @@ SVX_E_ARG, SVX_E_MAGIC, SVX_E_FORMAT, SVX_E_CAPACITY, SVX_E_HIP, SVX_E_NODEV = -
 SYMBOLS = ["svx_init", "svx_run_segment", "svx_run_segment_ex", "svx_engine", "svx_default_options",
            "svx_prepare", "svx_prepare_ex", "svx_release_prepared", "svx_cache_limit", "svx_prepared_shape",
            "svx_run_prepared", "svx_tune_prepared", "svx_prepared_info", "svx_prepared_kernel",
-           "svx_prepared_pass_stats", "svx_run_batch", "svx_permute", "svx_permute_plan", "svx_swap_words", "svx_zero_state",
+           "svx_prepared_pass_stats", "svx_run_batch", "svx_queue_open", "svx_queue_submit", "svx_queue_done",
+           "svx_queue_wait", "svx_queue_close", "svx_permute", "svx_permute_plan", "svx_swap_words", "svx_zero_state",
            "svx_product_state", "svx_reduce_f64", "svx_reduce_f64_noalloc", "svx_fidelity_f64", "svx_convert", "svx_sync",
            "svx_last_error", "svx_version"]
 
@@ -118,6 +119,11 @@ _SIGS = {
     "svx_prepared_kernel": ([_I64, _I32, ctypes.c_char_p, _I32], _I32),
     "svx_prepared_pass_stats": ([_I64, _I32, _PI32, ], _I32),
     "svx_run_batch": ([ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int64), _PI32, _I32, _VP], _I32),
+    "svx_queue_open": ([_I32, _I32, _I32, _I32, ctypes.POINTER(_I64)], _I32),
+    "svx_queue_submit": ([_I64, _VP, _I64, _I32, ctypes.POINTER(_I64)], _I32),
+    "svx_queue_done": ([_I64, _I64, _PI32], _I32),
+    "svx_queue_wait": ([_I64, _I64], _I32),
+    "svx_queue_close": ([_I64], _I32),
     "svx_permute": ([_VP, _VP, _I32, _PI32, _I32, _VP], _I32),
     "svx_permute_plan": ([_I32, _PI32, _I32, _PI32, _I32, _PI32], _I32),
     "svx_swap_words": ([_VP, _VP, _I64, _I64, _I64, _VP], _I32),
@@ -330,6 +336,36 @@ def prepared_pass_stats(handle, pass_index):
     _check(lib.svx_prepared_pass_stats(handle, pass_index, out))
     return dict(mux_groups=out[0], mux_ops=out[1], dense4=out[2], dense5=out[3], dense_exact=out[4],
                 mux_mfma=out[5], diag_tables=out[6], tile_order=out[7])
+
+
+class Queue:
+    """svx_queue_*: a persistent kernel running whole-state circuits of 2^state_bits amplitudes
+    (one prepared whole-state pass each) from a host-visible ring, many circuits per launch."""
+
+    def __init__(self, state_bits, capacity=1024, workgroups=64, idle_us=20000):
+        self.bits = state_bits
+        h = ctypes.c_int64(0)
+        _check(load().svx_queue_open(int(state_bits), int(capacity), int(workgroups), int(idle_us), ctypes.byref(h)))
+        self.h = h.value
+
+    def submit(self, state, handle, zero_init=True):
+        _check_state(state, handle)
+        t = ctypes.c_int64(0)
+        _check(load().svx_queue_submit(self.h, _dev(state), int(handle), int(bool(zero_init)), ctypes.byref(t)))
+        return t.value
+
+    def done(self, ticket):
+        d = ctypes.c_int32(0)
+        _check(load().svx_queue_done(self.h, int(ticket), ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        _check(load().svx_queue_wait(self.h, int(ticket)))
+
+    def close(self):
+        if self.h:
+            _check(load().svx_queue_close(self.h))
+            self.h = 0
 
 
 def permute(dst, src, perm, stream=None):

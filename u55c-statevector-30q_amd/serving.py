@@ -4,7 +4,9 @@ Requests carry compiled QSV2 programs (the client compiles; `compile_request`). 
 - admits a request when its state footprint 2^n x 8 B fits the HBM budget (288 GB, or the device's
   memory, minus a reserve) and keeps its complex64 state resident until the result is read;
 - runs circuits of up to 2^13 amplitudes as one whole-state pass (one workgroup holds the state in
-  LDS for the whole circuit) and larger ones pass by pass;
+  LDS for the whole circuit) on the library's persistent queue kernel (svx_queue_*: a resident launch
+  takes each such circuit from a host-visible ring, many circuits per launch, no launch per circuit),
+  and larger ones pass by pass;
 - advances every active batched circuit by one pass per step with `libsvx.run_batch` (one launch per
   kernel shape, the tiles of all circuits side by side in one grid), so requests that arrive while
   others run join at the next step and their work shares the device;
@@ -61,6 +63,8 @@ class Request:
     t_done: float = None
     marginals: np.ndarray = None
     norm: float = None
+    queue: object = None               # the persistent queue running it (mode "queue")
+    ticket: int = -1
 
     @property
     def footprint(self):
@@ -71,7 +75,7 @@ class SVServer:
     """Continuous-batching request loop (see the module docstring)."""
 
     def __init__(self, device=0, budget_bytes=None, reserve_bytes=16 << 30, graph_min_qubits=30,
-                 max_inflight_steps=2, keep_states=False):
+                 max_inflight_steps=2, keep_states=False, queue_small=True):
         import torch
         self.torch = torch
         self.caps = libsvx.init(device)
@@ -83,6 +87,9 @@ class SVServer:
         self.graph_min_qubits = graph_min_qubits
         self.max_inflight = max_inflight_steps
         self.keep_states = keep_states
+        self.queue_small = queue_small
+        self.queues = {}          # state bits -> libsvx.Queue (persistent whole-state kernel)
+        self.n_queued = 0
         self.s_batch = torch.cuda.Stream(device)
         self.s_graph = torch.cuda.Stream(device)
         self.s_red = torch.cuda.Stream(device)
@@ -117,7 +124,8 @@ class SVServer:
                     return
                 r = self._to_reduce.popleft()
             try:
-                self.s_red.wait_event(r.done_event)
+                if r.done_event is not None:
+                    self.s_red.wait_event(r.done_event)
                 fm = list(r.fmap[:N_MARGINAL])
                 r.norm, r.marginals = libsvx.reduce_f64(r.state, fm, stream=self.s_red.cuda_stream)
                 r.t_done = self.now()
@@ -167,7 +175,16 @@ class SVServer:
             self.in_use += req.footprint
             self.peak_in_use = max(self.peak_in_use, self.in_use)
             req.t_admit = self.now()
-            if req.n >= self.graph_min_qubits:
+            if self._queueable(req):
+                req.mode = "queue"
+                q = self.queues.get(req.n)
+                if q is None:
+                    q = self.queues[req.n] = libsvx.Queue(req.n, capacity=4096, workgroups=64)
+                req.queue = q
+                req.ticket = q.submit(req.state, req.steps[0][0], zero_init=True)
+                self.n_queued += 1
+                self.finishing.append(req)
+            elif req.n >= self.graph_min_qubits:
                 req.mode = "graph"
                 g = torch.cuda.CUDAGraph()
                 cap = torch.cuda.Stream(self.device)
@@ -183,6 +200,17 @@ class SVServer:
                 with torch.cuda.stream(self.s_batch):
                     libsvx.zero_state(req.state, stream=self.s_batch.cuda_stream)
                 self.active.append(req)
+
+    def _queueable(self, req):
+        """One whole-state pass on the MUX2 kernel, 2^10 .. 2^13 amplitudes: the queue kernel runs it."""
+        if not self.queue_small or not 10 <= req.n <= WHOLE_STATE_MAX or len(req.steps) != 1:
+            return False
+        return libsvx.prepared_kernel(req.steps[0][0], 0).startswith("k_mux<")
+
+    def close(self):
+        for q in self.queues.values():
+            q.close()
+        self.queues = {}
 
     @staticmethod
     def _release(req):
@@ -237,9 +265,16 @@ class SVServer:
         reduced ones: free their states and give back their handles.  `block`: wait for one."""
         left = []
         for r in self.finishing:
-            if block:
-                r.done_event.synchronize()
-            if r.done_event.query():
+            if r.mode == "queue":
+                ready = r.queue.done(r.ticket)
+                if block and not ready:
+                    r.queue.wait(r.ticket)
+                    ready = True
+            else:
+                if block:
+                    r.done_event.synchronize()
+                ready = r.done_event.query()
+            if ready:
                 with self._cv:
                     self._to_reduce.append(r)
                     self._handed += 1
@@ -277,6 +312,7 @@ class SVServer:
             self._loop(pending, realtime)
         finally:
             self._stop_collector()
+            self.close()
         self.torch.cuda.synchronize(self.device)
         return self.done
 
