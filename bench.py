@@ -27,6 +27,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line on stderr as each section of the run completes (the JSON line stays alone on stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {what}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(max_seconds=30.0):
@@ -346,7 +352,7 @@ def fused_section(libsvx, circuit, runtime, torch, np, n, arch, judge=True):
     return out
 
 
-def served_section(serving, torch, seconds=20.0, seed=3, burst=60):
+def served_section(serving, torch, seconds=20.0, seed=3, burst=60, progress_every=0):
     """Served mix: capacity from a calibration burst (requests all at time 0, another seed), then a
     Poisson stream at 75% of it for `seconds`; circuits/s, p50/p99 latency (arrival to FP64
     marginals on the host), and the complex128 plain fallback as judge on a sample."""
@@ -357,10 +363,13 @@ def served_section(serving, torch, seconds=20.0, seed=3, burst=60):
     done = srv.serve(cal, realtime=False)
     cap = serving.latency_stats(done)["circuits_per_s"]
     rate = 0.75 * cap
-    reqs = serving.make_stream(rate=rate, duration=seconds, seed=seed, compile_cache=cache)
+    reqs = serving.make_stream(rate=rate, duration=seconds, seed=seed, compile_cache=cache,
+                               progress_every=10 * progress_every)
     t_compile = time.perf_counter() - t0
+    if progress_every:
+        progress(f"served mix: {len(reqs)} requests over {seconds:.0f} s at {rate:.2f}/s, compiled in {t_compile:.1f} s")
     srv = serving.SVServer()
-    done = srv.serve(reqs, realtime=True)
+    done = srv.serve(reqs, realtime=True, progress_every=progress_every)
     st = serving.latency_stats(done)
     judge = serving.judge_c128(done, every=10, max_qubits=24)
     big = [r for r in done if r.n >= 30]
@@ -438,7 +447,8 @@ def main():
 
     caps = libsvx.init(0)
     if a.serve_only:
-        print(json.dumps({"served_mix": served_section(serving, torch, seconds=a.serve_seconds)}))
+        print(json.dumps({"served_mix": served_section(serving, torch, seconds=a.serve_seconds,
+                                                       progress_every=250 if a.serve_seconds > 60 else 0)}))
         return
     n = 30
     qc = circuit.workload_gpu30()
@@ -523,6 +533,7 @@ def main():
         e1.record(stream)
     torch.cuda.synchronize()
     per_pass_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    progress(f"30-qubit workload: {ms_per_step:.3f} ms per step, {value:.1f} GB/s; per-pass timing done")
     kernels = [libsvx.prepared_kernel(h, 0) for h in single]
     by_kernel = {}
     for kname, t in zip(kernels, per_pass_ms):
@@ -548,21 +559,30 @@ def main():
         del s64, s128
         torch.cuda.empty_cache()
 
+    progress(f"judge vs complex128: {judge}")
     ladder = None if a.no_judge else cp_ladder_section(libsvx, circuit, compiler, torch, np)
+    progress("controlled-phase ladder done")
     arch = caps.arch.decode().split(":")[0]
     del state
     torch.cuda.empty_cache()
     fused = None
     if not a.no_fused:
-        fused = {str(nq): fused_section(libsvx, circuit, runtime, torch, np, nq, arch, judge=(nq == 32 and not a.no_judge))
-                 for nq in (32, 34)}
+        fused = {}
+        for nq in (32, 34):
+            fused[str(nq)] = fused_section(libsvx, circuit, runtime, torch, np, nq, arch,
+                                           judge=(nq == 32 and not a.no_judge))
+            progress(f"fused vs plain at {nq} qubits done")
     families = None if a.no_families else families_section(runtime, circuit, torch, np)
+    progress("30-qubit families done")
     latency = None if a.no_latency else latency_section(runtime, circuit, np, runs=a.latency_runs)
+    progress("serial latency done")
     roof = hbm_roof_section(libsvx, torch)
     served = None if a.no_serve else served_section(serving, torch, seconds=a.serve_seconds)
+    progress("served mix done")
     cpu = None
     if not a.no_cpu:
         cpu = _run_cpu_child()
+        progress("cpu baseline done")
         cpu["gpu_same_records"] = gpu_same_records(libsvx, circuit, compiler, torch, np)
     out = {
         "metric": "30q c64 pass GB/s (% of 8 TB/s); fused vs plain time; served circuits/s+p99",

@@ -22,6 +22,7 @@ Results do not depend on the mix: a batched pass runs the same device code as a 
 """
 import collections
 import threading
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -107,6 +108,10 @@ class SVServer:
         self._to_reduce = collections.deque()
         self.reduced = collections.deque()
         self._cv = threading.Condition()
+        # the collector's HIP calls (reduction, read-back, scratch growth) and the loop thread's graph
+        # captures never overlap: a capture is invalidated by another thread's allocating or
+        # synchronising call on some HIP versions, whatever the capture mode
+        self._hip_lock = threading.Lock()
         self._stop = False
         self._collector = None
         self.collector_error = None
@@ -124,10 +129,11 @@ class SVServer:
                     return
                 r = self._to_reduce.popleft()
             try:
-                if r.done_event is not None:
-                    self.s_red.wait_event(r.done_event)
                 fm = list(r.fmap[:N_MARGINAL])
-                r.norm, r.marginals = libsvx.reduce_f64(r.state, fm, stream=self.s_red.cuda_stream)
+                with self._hip_lock:
+                    if r.done_event is not None:
+                        self.s_red.wait_event(r.done_event)
+                    r.norm, r.marginals = libsvx.reduce_f64(r.state, fm, stream=self.s_red.cuda_stream)
                 r.t_done = self.now()
             except Exception as e:   # reported by the loop thread
                 self.collector_error = e
@@ -188,7 +194,8 @@ class SVServer:
                 req.mode = "graph"
                 g = torch.cuda.CUDAGraph()
                 cap = torch.cuda.Stream(self.device)
-                with torch.cuda.graph(g, stream=cap):
+                # the collector thread makes no HIP call while this thread captures
+                with self._hip_lock, torch.cuda.graph(g, stream=cap, capture_error_mode="thread_local"):
                     s = torch.cuda.current_stream().cuda_stream
                     libsvx.zero_state(req.state, stream=s)
                     for h, p in self._segments(req):
@@ -296,15 +303,20 @@ class SVServer:
             r.graph = None            # replayed once; its handles are given back below
             self._release(r)
             self.done.append(r)
+            if getattr(self, "progress_every", 0) and len(self.done) % self.progress_every == 0:
+                print(f"[serve {self.now():7.1f} s] {len(self.done)} circuits done, {self.n_steps} batched steps",
+                      file=sys.stderr, flush=True)
 
     def outstanding(self):
         """Circuits handed to the collector whose results the loop has not taken back yet."""
         return self._handed - self._taken
 
-    def serve(self, requests, realtime=True):
+    def serve(self, requests, realtime=True, progress_every=0):
         """Run a request stream: each request is submitted at its arrival time (seconds from the
         start; all at once when realtime is False) and the loop steps until every result is in.
-        Returns the completed requests."""
+        progress_every > 0: one line on stderr each time that many more results are in.  Returns
+        the completed requests."""
+        self.progress_every = int(progress_every)
         pending = sorted(requests, key=lambda r: r.arrival)
         self.t0 = time.perf_counter()
         self._start_collector()
@@ -349,10 +361,12 @@ def mix_circuit(rng, rid):
     return circuit_mod.u3_cx_layers(n, depth, 3000 + 100 * n + depth), n, depth
 
 
-def make_stream(n_requests=None, rate=None, duration=None, seed=3, max_qubits=32, compile_cache=None):
+def make_stream(n_requests=None, rate=None, duration=None, seed=3, max_qubits=32, compile_cache=None,
+                progress_every=0):
     """Poisson arrivals at `rate` per second for `duration` seconds (or `n_requests` requests at
     time 0 when rate is None), each a compiled circuit of the mix.  Requests above max_qubits are
-    redrawn (for boxes with less memory)."""
+    redrawn (for boxes with less memory).  progress_every > 0: a line on stderr per that many
+    requests made."""
     rng = np.random.default_rng(seed)
     reqs = []
     t = 0.0
@@ -377,6 +391,8 @@ def make_stream(n_requests=None, rate=None, duration=None, seed=3, max_qubits=32
         reqs.append(Request(rid=rid, n=n, prog=prog, fmap=fmap, arrival=t if rate is not None else 0.0,
                             depth=depth))
         rid += 1
+        if progress_every and rid % progress_every == 0:
+            print(f"[stream] {rid} requests compiled ({len(compile_cache or {})} distinct)", file=sys.stderr, flush=True)
     return reqs
 
 
