@@ -202,6 +202,14 @@ __device__ __forceinline__ C2<float> ldc(const unsigned char *buf, uint32_t a) {
 __device__ __forceinline__ void stc(unsigned char *buf, uint32_t a, C2<float> v) {
     *reinterpret_cast<C2<float> *>(buf + a) = v;
 }
+// re and im of one complex64 tile element as two 32-bit LDS stores (paired into one ds_write2 with
+// no register moves, where a 64-bit store would need the two values in adjacent registers)
+__device__ __forceinline__ void stc2(unsigned char *buf, uint32_t a, float re, float im) {
+    float *p = reinterpret_cast<float *>(buf + a);
+    p[0] = re;
+    p[1] = im;
+}
+
 
 // FP32 matrix cores (exact for 0 / +-1 matrices): v_mfma_f32_32x32x2f32, half h = l / 32 feeds K-rows
 // 2s + h; W arrives in A-operand order wd[(rt 2^F + s) 64 + l] = W[32 rt + l % 32][2 s + l / 32].
@@ -213,7 +221,8 @@ __device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &
     constexpr int NS = 1 << F;                     // K-steps of 2 real rows
     constexpr int NX = NS / 2;                     // complex inputs per lane
     constexpr uint32_t NB = (1u << (K - F)) / 32;  // 32-column batches per tile
-    const uint32_t lane = wt & 63u, wave = wt >> 6, h = lane >> 5, j = lane & 31u;
+    const uint32_t lane = wt & 63u, h = lane >> 5, j = lane & 31u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(wt >> 6);
     const float *w = wd + G.w_off;
     float wa[NRT][NS];
 #pragma unroll
@@ -245,13 +254,13 @@ __device__ __forceinline__ void dense_apply(unsigned char *buf, const GroupDev &
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[0][v + 8]});
+                stc2(buf, TileAddr<SWZ>::comb(pb, coff(c)), acc[0][v], acc[0][v + 8]);
             }
         } else {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const uint32_t c = 8u * (v / 4) + 4u * h + (v % 4);
-                stc(buf, TileAddr<SWZ>::comb(pb, coff(c)), C2<float>{acc[0][v], acc[NRT - 1][v]});
+                stc2(buf, TileAddr<SWZ>::comb(pb, coff(c)), acc[0][v], acc[NRT - 1][v]);
             }
         }
     }
@@ -267,11 +276,16 @@ typedef float f4acc __attribute__((ext_vector_type(4)));
 // 4 g .. 4 g + 3 of column l % 16 and writes the same four outputs (no other lane touches them).
 // Operands: wd[(s 4 + t) 64 + l] = W_s[l % 16][4 (l / 16) + t] with W_0 = Wr + Wi, W_1 = -Wi, W_2 = Wr.
 // Each wavefront step carries two 16-column blocks, two independent MFMA chains.
+// Addresses: column c = 16 blk + j of the tile deposits its bits on the tile bits outside the unit's
+// four (ascending).  The deposit of disjoint parts is the sum (padded layout) or XOR (swizzled) of
+// their deposits, so a lane's part (j, and its components' offsets) is formed once per unit and the
+// block part (wavefront-uniform) in scalar registers per block.
 template <int K, int NWAVE, bool SWZ>
 __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                              uint32_t wt) {
     constexpr uint32_t NB = (1u << (K - 4)) / 16;   // 16-column blocks per tile
-    const uint32_t lane = wt & 63u, wave = wt >> 6, g = lane >> 4, j = lane & 15u;
+    const uint32_t lane = wt & 63u, g = lane >> 4, j = lane & 15u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(wt >> 6);
     const float *w = wd + G.w_off;
     float w0[4], w1[4], w2[4];
 #pragma unroll
@@ -280,24 +294,27 @@ __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev 
         w1[t] = w[(4 + t) * 64 + lane];
         w2[t] = w[(8 + t) * 64 + lane];
     }
+    auto dep = [&](uint32_t c) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c = ins0_32(c, G.dqs[i]);
+        return c;
+    };
     ColOff<4, SWZ> coff;
     coff.init(G);
+    const uint32_t lane_off = TileAddr<SWZ>::of(dep(j));
     uint32_t co[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) co[t] = coff(4u * g + t);
+    for (int t = 0; t < 4; ++t) co[t] = TileAddr<SWZ>::comb(lane_off, coff(4u * g + t));
     for (uint32_t b = wave; b < NB; b += 2 * NWAVE) {
-        uint32_t pb[2];
-        C2<float> x[2][4];
         // the second block of a step, when the tile has one (else block b again, not stored)
         const bool two = b + NWAVE < NB;
+        uint32_t ub[2];
+        C2<float> x[2][4];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            uint32_t base = (b + (h && two ? NWAVE : 0)) * 16u + j;
+            ub[h] = TileAddr<SWZ>::of(dep((b + (h && two ? NWAVE : 0)) * 16u));
 #pragma unroll
-            for (int i = 0; i < 4; ++i) base = ins0_32(base, G.dqs[i]);
-            pb[h] = TileAddr<SWZ>::of(base);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) x[h][t] = ldc(buf, TileAddr<SWZ>::comb(pb[h], co[t]));
+            for (int t = 0; t < 4; ++t) x[h][t] = ldc(buf, TileAddr<SWZ>::comb(co[t], ub[h]));
         }
         f4acc k1[2], re[2], im[2];
 #pragma unroll
@@ -317,10 +334,10 @@ __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev 
             }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) stc(buf, TileAddr<SWZ>::comb(pb[0], co[v]), C2<float>{re[0][v], im[0][v]});
+        for (int v = 0; v < 4; ++v) stc2(buf, TileAddr<SWZ>::comb(co[v], ub[0]), re[0][v], im[0][v]);
         if (two)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) stc(buf, TileAddr<SWZ>::comb(pb[1], co[v]), C2<float>{re[1][v], im[1][v]});
+            for (int v = 0; v < 4; ++v) stc2(buf, TileAddr<SWZ>::comb(co[v], ub[1]), re[1][v], im[1][v]);
     }
 }
 
