@@ -280,19 +280,26 @@ typedef float f4acc __attribute__((ext_vector_type(4)));
 // four (ascending).  The deposit of disjoint parts is the sum (padded layout) or XOR (swizzled) of
 // their deposits, so a lane's part (j, and its components' offsets) is formed once per unit and the
 // block part (wavefront-uniform) in scalar registers per block.
+// the lane's twelve A-operand values of a 4-qubit three-product unit (loaded ahead of the unit by
+// callers that can, so the loads overlap a barrier or the tile load)
+struct Gauss4W {
+    float w[12];
+};
+__device__ __forceinline__ void gauss4_weights(const float *__restrict__ w, uint32_t lane, Gauss4W &out) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out.w[i] = w[i * 64 + (lane & 63u)];
+}
 template <int K, int NWAVE, bool SWZ>
-__device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
-                                             uint32_t wt) {
+__device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev &G, const Gauss4W &gw, uint32_t wt) {
     constexpr uint32_t NB = (1u << (K - 4)) / 16;   // 16-column blocks per tile
     const uint32_t lane = wt & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(wt >> 6);
-    const float *w = wd + G.w_off;
     float w0[4], w1[4], w2[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        w0[t] = w[t * 64 + lane];
-        w1[t] = w[(4 + t) * 64 + lane];
-        w2[t] = w[(8 + t) * 64 + lane];
+        w0[t] = gw.w[t];
+        w1[t] = gw.w[4 + t];
+        w2[t] = gw.w[8 + t];
     }
     auto dep = [&](uint32_t c) {
 #pragma unroll
@@ -347,7 +354,11 @@ __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev 
 template <int K, int NWAVE, bool SWZ>
 __device__ __forceinline__ void dense_group_f32(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                                 uint32_t wt) {
-    if (G.type == 5) dense_gauss4<K, NWAVE, SWZ>(buf, G, wd, wt);
+    if (G.type == 5) {
+        Gauss4W gw;
+        gauss4_weights(wd + G.w_off, wt, gw);
+        dense_gauss4<K, NWAVE, SWZ>(buf, G, gw, wt);
+    }
     else if (G.f == 5) dense_apply<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
     else dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
 }

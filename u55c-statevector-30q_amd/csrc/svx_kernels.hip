@@ -582,7 +582,9 @@ template <int K, int NWAVE, bool SWZ>
 __device__ __forceinline__ void dense_group(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                             uint32_t wt) {
     if (G.type == 5) {
-        dense_gauss4<K, NWAVE, SWZ>(buf, G, wd, wt);
+        Gauss4W gw;
+        gauss4_weights(wd + G.w_off, wt, gw);
+        dense_gauss4<K, NWAVE, SWZ>(buf, G, gw, wt);
     } else if (G.type == 1) {
         if (G.f == 5) dense_apply<K, 5, NWAVE, SWZ>(buf, G, wd, wt);
         else dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
@@ -600,7 +602,9 @@ template <int K, int NWAVE, bool SWZ = false>
 __device__ __forceinline__ void dense_group4(unsigned char *buf, const GroupDev &G, const float *__restrict__ wd,
                                              uint32_t wt) {
     if (G.type == 5) {
-        dense_gauss4<K, NWAVE, SWZ>(buf, G, wd, wt);
+        Gauss4W gw;
+        gauss4_weights(wd + G.w_off, wt, gw);
+        dense_gauss4<K, NWAVE, SWZ>(buf, G, gw, wt);
     } else if (G.type == 1) {
         dense_apply<K, 4, NWAVE, SWZ>(buf, G, wd, wt);
     } else if (G.type == 4) {

@@ -363,23 +363,43 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
     }
     __syncthreads();
     const int lowb = k - m;
+    const int ng = pd->ngroups, g0 = pd->group0;
+    // a three-product dense unit's A operands are loaded one group ahead (before the barrier that
+    // precedes the unit, or with the tile), so their latency overlaps the barrier or the tile load
+    Gauss4W gw;
+    int gw_g = -1;   // the group whose operands gw holds
+    auto prefetch = [&](int g) {
+        if constexpr (DN) {
+            if (g < ng && gd[g0 + g].type == 5) {
+                gauss4_weights(wd + gd[g0 + g].w_off, tid, gw);
+                gw_g = g;
+            }
+        }
+    };
+    prefetch(0);
     if (zero) {
         for (uint32_t e = tid; e < NE; e += NT) tile[lpad(e)] = C2<float>{e == 0 ? 1.0f : 0.0f, 0.0f};
     } else {
         move_tile<float, K, true, NT>(st, tile, sbo, tb, k, lowb, tid);
     }
     __syncthreads();
-    const int ng = pd->ngroups, g0 = pd->group0;
     for (int g = 0; g < ng; ++g) {
         const GroupDev &G = gd[g0 + g];
         if (G.type == 6) {
             diag_table<K, NT>(tile, wd + G.w_off, k, tid);
+            prefetch(g + 1);
             __syncthreads();
             continue;
         }
         if constexpr (DN) {
             if (G.type != 0) {
-                dense_group_f32<K, NT / 64, false>(smem, G, wd, tid);
+                if (G.type == 5) {
+                    if (gw_g != g) prefetch(g);
+                    dense_gauss4<K, NT / 64, false>(smem, G, gw, tid);
+                } else {
+                    dense_group_f32<K, NT / 64, false>(smem, G, wd, tid);
+                }
+                prefetch(g + 1);
                 __syncthreads();
                 continue;
             }
@@ -419,6 +439,7 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
 #pragma unroll
             for (int r = 0; r < NR; ++r) tile[pb + poff[r]] = C2<float>{A[2 * r], A[2 * r + 1]};
         }
+        prefetch(g + 1);
         __syncthreads();
     }
     move_tile<float, K, false, NT>(st, tile, sbo, tb, k, lowb, tid);
