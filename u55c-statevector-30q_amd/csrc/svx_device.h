@@ -275,7 +275,7 @@ typedef float f4acc __attribute__((ext_vector_type(4)));
 // K step t, lane quarter g = l / 16 carries component 4 g + t, so a lane reads components
 // 4 g .. 4 g + 3 of column l % 16 and writes the same four outputs (no other lane touches them).
 // Operands: wd[(s 4 + t) 64 + l] = W_s[l % 16][4 (l / 16) + t] with W_0 = Wr + Wi, W_1 = -Wi, W_2 = Wr.
-// Each wavefront step carries two 16-column blocks, two independent MFMA chains.
+// Each wavefront step carries up to two 16-column blocks, independent MFMA chains.
 // Addresses: column c = 16 blk + j of the tile deposits its bits on the tile bits outside the unit's
 // four (ascending).  The deposit of disjoint parts is the sum (padded layout) or XOR (swizzled) of
 // their deposits, so a lane's part (j, and its components' offsets) is formed once per unit and the
@@ -312,39 +312,46 @@ __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev 
     uint32_t co[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) co[t] = TileAddr<SWZ>::comb(lane_off, coff(4u * g + t));
-    for (uint32_t b = wave; b < NB; b += 2 * NWAVE) {
-        // the second block of a step, when the tile has one (else block b again, not stored)
-        const bool two = b + NWAVE < NB;
-        uint32_t ub[2];
-        C2<float> x[2][4];
+    // BPS blocks per wavefront step (independent MFMA chains): the tile's blocks split evenly over
+    // the wavefronts, at most two per step (four measured slower)
+    constexpr uint32_t PER = NB / NWAVE > 0 ? NB / NWAVE : 1;
+    constexpr uint32_t BPS = PER < 2 ? PER : 2;
+    for (uint32_t b = wave; b < NB; b += BPS * NWAVE) {
+        uint32_t ub[BPS];
+        bool ok[BPS];
+        C2<float> x[BPS][4];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            ub[h] = TileAddr<SWZ>::of(dep((b + (h && two ? NWAVE : 0)) * 16u));
+        for (uint32_t h = 0; h < BPS; ++h) {
+            const uint32_t blk = b + h * NWAVE;
+            ok[h] = blk < NB;   // uniform
+            ub[h] = TileAddr<SWZ>::of(dep((ok[h] ? blk : b) * 16u));
 #pragma unroll
             for (int t = 0; t < 4; ++t) x[h][t] = ldc(buf, TileAddr<SWZ>::comb(co[t], ub[h]));
         }
-        f4acc k1[2], re[2], im[2];
+        f4acc k1[BPS], re[BPS], im[BPS];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            k1[h] = f4acc{0, 0, 0, 0};
+        for (uint32_t h = 0; h < BPS; ++h) k1[h] = f4acc{0, 0, 0, 0};
 #pragma unroll
-            for (int t = 0; t < 4; ++t) k1[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w0[t], x[h][t].x, k1[h], 0, 0, 0);
-        }
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+            for (uint32_t h = 0; h < BPS; ++h) k1[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w0[t], x[h][t].x, k1[h], 0, 0, 0);
+#pragma unroll
+        for (uint32_t h = 0; h < BPS; ++h) {
             re[h] = k1[h];
             im[h] = k1[h];
+        }
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (uint32_t h = 0; h < BPS; ++h) {
                 re[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w1[t], x[h][t].x + x[h][t].y, re[h], 0, 0, 0);
                 im[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(w2[t], x[h][t].y - x[h][t].x, im[h], 0, 0, 0);
             }
-        }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) stc2(buf, TileAddr<SWZ>::comb(co[v], ub[0]), re[0][v], im[0][v]);
-        if (two)
+        for (uint32_t h = 0; h < BPS; ++h)
+            if (ok[h])
 #pragma unroll
-            for (int v = 0; v < 4; ++v) stc2(buf, TileAddr<SWZ>::comb(co[v], ub[1]), re[1][v], im[1][v]);
+                for (int v = 0; v < 4; ++v) stc2(buf, TileAddr<SWZ>::comb(co[v], ub[h]), re[h][v], im[h][v]);
     }
 }
 
