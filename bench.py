@@ -426,7 +426,8 @@ def main():
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--ladder-probe", action="store_true",
                     help="for rocprofv3 kernel traces: the 30-qubit controlled-phase ladder's complex64 passes "
-                         "(tile orders tuned first), once, then the bit reversal twice; prints per-pass work")
+                         "(tile orders tuned first), once, then the bit reversal twice out of place and once in "
+                         "place; prints per-pass work")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: an 8 GiB complex64 copy (svx_convert, calibration) twice, "
                          "the tile-order tuning launches (on the copy's destination), then one step of the "
@@ -468,9 +469,11 @@ def main():
             libsvx.run_prepared(s64, h)
         for _ in range(2):
             libsvx.permute(t64, s64, list(range(n))[::-1])
+        libsvx.permute(t64, t64, list(range(n))[::-1])   # in place: a planned sequence of tile-local sweeps
         torch.cuda.synchronize()
         work = [dict(libsvx.prepared_pass_stats(h, 0), kernel=libsvx.prepared_kernel(h, 0)) for h in hs]
-        print(json.dumps({"passes": len(hs), "per_pass": work}))
+        print(json.dumps({"passes": len(hs), "per_pass": work,
+                          "in_place_sweeps": len(libsvx.permute_plan(list(range(n))[::-1]))}))
         return
     if a.traffic_probe:
         prog, fmap, st = compiler.compile_gpu(qc)
