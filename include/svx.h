@@ -39,14 +39,15 @@ extern "C" {
 #define SVX_FUSE_AUTO 0      /* cluster a pass's MUX2 ops into dense units where the units are cheaper */
 #define SVX_FUSE_OFF 1       /* every op on the MUX2 path */
 #define SVX_FUSE_DENSE4 2    /* every block of <= 4 tile bits a dense 4-qubit unit */
-#define SVX_DENSE_F32 0      /* dense units on FP32 matrix cores (v_mfma_f32_32x32x2f32): default */
+#define SVX_DENSE_F32 0      /* dense units on FP32 matrix cores, default: 4-qubit units as three real
+                                16x16 products on v_mfma_f32_16x16x4f32, 5-qubit units and units whose
+                                entries are all 0, +-1, +-i as the real form on v_mfma_f32_32x32x2f32 */
 #define SVX_DENSE_SPLIT16 1  /* opt-in: split-FP16 matrix cores (hi 11 bits + lo FP16, lo x lo dropped,
                                 block-scaled below 2^15): ~2^-21 error per product relative to the
                                 tile/column maximum, so an amplitude 1e-9 below its tile maximum keeps
                                 only ~3 significant digits; faster than FP32 matrix cores */
 #define SVX_KERNEL_AUTO 0    /* per-pass choice (single-buffer / dense-only kernels) */
-#define SVX_KERNEL_SINGLE 1  /* always the single-buffer pass kernel */
-#define SVX_KERNEL_DMA 2     /* double-buffered pass kernel (loader wave + workers), complex64, k in {12,13} */
+#define SVX_KERNEL_SINGLE 1  /* the same choice (kept for callers that set it) */
 #define SVX_ORDER_TUNABLE 0  /* XCD-contiguous tile order; svx_tune_prepared may switch passes to identity */
 #define SVX_ORDER_IDENTITY 1 /* fixed identity workgroup -> tile order */
 #define SVX_ORDER_XCD 2      /* fixed XCD-contiguous order (not tunable) */
@@ -153,7 +154,7 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
 /* Number of passes (kernel launches) of a prepared segment, and the kernel index of pass p. */
 int svx_prepared_info(int64_t handle, int32_t *n_passes, int32_t *n_launches);
 
-/* Name of the kernel that runs pass `pass` of a prepared segment ("k_pass<float,13>", "k_pass_dma<13,4>",
+/* Name of the kernel that runs pass `pass` of a prepared segment ("k_mux<13,4>", "k_pass<double,13,false>",
  * ..., "none" for a pass without effective ops), for profiling reports.  No reference counterpart. */
 int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
 

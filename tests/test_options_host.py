@@ -7,9 +7,9 @@ from u55c_statevector_30q_amd import libsvx
 
 
 def test_make_options_maps_names_to_the_header_constants():
-    o = libsvx.make_options(fuse="off", dense_math="split16", kernel="dma", tile_order="identity",
+    o = libsvx.make_options(fuse="off", dense_math="split16", kernel="single", tile_order="identity",
                             mux_engine="valu", fuse_width=5, fuse_cost4=20.0)
-    assert (o.fuse, o.dense_math, o.kernel, o.tile_order, o.mux_engine) == (1, 1, 2, 1, 1)
+    assert (o.fuse, o.dense_math, o.kernel, o.tile_order, o.mux_engine) == (1, 1, 1, 1, 1)
     assert o.size == 4 * 16 and o.fuse_width == 5 and abs(o.fuse_cost4 - 20.0) < 1e-6
     d = libsvx.make_options()
     assert (d.fuse, d.dense_math, d.kernel) == (0, 0, 0), "FP32 dense units are the default"

@@ -84,7 +84,7 @@ int normalize_opts(const svx_options *in, Opts *o) {
     if (s.fuse < SVX_FUSE_AUTO || s.fuse > SVX_FUSE_DENSE4) return fail(SVX_E_ARG, "bad svx_options.fuse");
     if (s.dense_math != SVX_DENSE_F32 && s.dense_math != SVX_DENSE_SPLIT16)
         return fail(SVX_E_ARG, "bad svx_options.dense_math");
-    if (s.kernel < SVX_KERNEL_AUTO || s.kernel > SVX_KERNEL_DMA) return fail(SVX_E_ARG, "bad svx_options.kernel");
+    if (s.kernel < SVX_KERNEL_AUTO || s.kernel > SVX_KERNEL_SINGLE) return fail(SVX_E_ARG, "bad svx_options.kernel");
     if (s.tile_order < SVX_ORDER_TUNABLE || s.tile_order > SVX_ORDER_XCD)
         return fail(SVX_E_ARG, "bad svx_options.tile_order");
     if (s.mux_engine != SVX_MUX_AUTO && s.mux_engine != SVX_MUX_VALU) return fail(SVX_E_ARG, "bad svx_options.mux_engine");
@@ -352,7 +352,6 @@ bool same(const T *a, const T *b) {
 
 struct LaunchInfo {
     bool small;
-    bool dma;             // double-buffered complex64 kernel (k_pass_dma)
     int K;
     uint64_t ntiles;
     uint32_t grid;
@@ -973,11 +972,9 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
                                         std::to_string(maxK));
     const bool small = nbits <= kSmallMaxBits;
     // kernel choice per pass (SVX_KERNEL_AUTO): dense-only passes of 4-qubit units -> k_pass_d4,
-    // everything else -> the single-buffer kernel k_pass (two workgroups per CU, non-temporal 16-byte
-    // HBM access, register groups, dense groups on the matrix cores).  SVX_KERNEL_DMA selects the
-    // double-buffered kernel (loader wave + worker waves; complex64, k in {12, 13}).
-    const bool pick_dma = opt.kernel == SVX_KERNEL_DMA;
-    const bool pick_single = opt.kernel == SVX_KERNEL_SINGLE;
+    // everything else -> the single-buffer kernels (k_mux for complex64 MUX2 / FP32 passes, k_pass
+    // otherwise: two workgroups per CU, non-temporal 16-byte HBM access, register groups, dense
+    // groups on the matrix cores).
     const bool dense_f32 = opt.dense_math == SVX_DENSE_F32;
     const bool tile_scale_ok = opt.tile_scale != 0;
     FuseCfg fcfg;
@@ -990,9 +987,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
     }
     const int torder = opt.tile_order == SVX_ORDER_IDENTITY ? 0 : -2;   // default: each XCD a contiguous range
     const bool diag_ok = opt.diag != 0;
-    const bool dma_ok = dtype == SVX_C64 && (k == 12 || k == 13) && nbits >= k + 9;
-    if (pick_dma && !dma_ok && !small)
-        return fail(SVX_E_ARG, "SVX_KERNEL_DMA needs complex64, k in {12, 13} and >= 2^(k+9) amplitudes");
     int Kv1 = 0;
     if (!small) {
         Kv1 = std::max(k, kMinK);
@@ -1066,7 +1060,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             cost += kOpCost[od.kind];
             pops.push_back(od);
         }
-        if (dtype == SVX_C64 && !small && diag_ok && opt.fuse != SVX_FUSE_DENSE4 && !pick_dma) {
+        if (dtype == SVX_C64 && !small && diag_ok && opt.fuse != SVX_FUSE_DENSE4) {
             diag_runs(pops, dense_of, dus, k, kDiagRunCost);
             for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
@@ -1074,19 +1068,15 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             fuse_pass(pops, dense_of, dus, fcfg);
             for (int d : dense_of) has_dense = has_dense || d >= 0;
         }
-        int mode;   // 0 small, 1 single-buffer, 2 DMA
-        if (small) mode = 0;
-        else if (pick_dma && dma_ok) mode = 2;
-        else mode = 1;   // light and heavy passes alike: the single-buffer kernel
-        (void)pick_single;
-        const int K = mode == 0 ? nbits : (mode == 1 ? Kv1 : k);
+        const int mode = small ? 0 : 1;   // 0 small-state kernel, 1 single-buffer kernels
+        const int K = mode == 0 ? nbits : Kv1;
         // complex64 passes without dense units (diagonal tables allowed) run on the FP32 MUX2 kernel
         bool has_units = false;
         for (size_t i = 0; i < dense_of.size(); ++i) has_units = has_units || (dense_of[i] >= 0 && !dus[dense_of[i]].diag);
         // the MUX2 kernel: complex64 passes of MUX2 ops and tables, and of FP32 dense units too (the
         // split-FP16 units need the tile scaling of k_pass)
         const bool use_mux = mode == 1 && dtype == SVX_C64 && (!has_units || dense_f32);
-        const int R = mode == 2 ? 4 : (use_mux ? mux_register_bits(K) : kR);
+        const int R = use_mux ? mux_register_bits(K) : kR;
         PassDev pd{};
         pd.k = k;
         pd.m = rp.m;
@@ -1095,25 +1085,12 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
         pd.torder = torder;
         for (int i = 0; i < rp.m; ++i) pd.h[i] = pd.hs[i] = rp.h[i];
         std::sort(pd.hs, pd.hs + rp.m);
-        if (mode == 2) {
-            // uniform part of the DMA loader's element offsets: tile elements 128 j + (< 128)
-            const int lowb = k - rp.m;
-            for (int j = 0; j < (1 << k) / 128 && j < 64; ++j) {
-                const uint64_t e0 = uint64_t(128) * j;
-                uint64_t u = e0 & ((uint64_t(1) << lowb) - 1);
-                const uint64_t sb = e0 >> lowb;
-                for (int i = 0; i < rp.m; ++i)
-                    if ((sb >> i) & 1) u |= uint64_t(1) << rp.h[i];
-                pd.dma_u[j] = u;
-            }
-        }
         pd.group0 = (int)groups.size();
         pd.op0 = (int)ops.size();
         pd.nops = (int)pops.size();
         LaunchInfo li{};
         li.small = mode == 0;
         li.dense = has_dense;
-        li.dma = mode == 2;
         li.K = K;
         li.pass_index = (int64_t)passes.size();
         li.empty = pd.nops == 0;
@@ -1125,11 +1102,6 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
             li.grid = 1;
             li.block = kNT;
             li.lds = 0;
-        } else if (mode == 2) {
-            const uint64_t per_cu = K == 12 ? 2 : 1;   // LDS: two 2^K complex64 buffers per workgroup
-            li.grid = (uint32_t)std::min<uint64_t>(li.ntiles, per_cu * (uint64_t)ds->caps.cu_count);
-            li.block = 64 + (K == 12 ? 256 : 512);
-            li.lds = pass_dma_lds_bytes(K, rp.m);
         } else {
             // one workgroup per tile: the grid in work-items must stay below 2^32 (HIP's limit)
             if (li.ntiles * kNTD4 > 0xffffffffull)
@@ -1634,9 +1606,6 @@ hipError_t launch_one(void *state, const Prepared *pr, const LaunchInfo &li, hip
     if (li.d4)
         return launch_pass_d4(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
                               li.grid, li.lds, s);
-    if (li.dma)
-        return launch_pass_dma(li.K, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
-                               li.ntiles, li.grid, li.lds, s);
     if (li.mux)
         return launch_mux(li.K, li.units, state, pd, gd, reinterpret_cast<const OpDev<float> *>(base + pr->op_off), wd,
                           li.grid, li.lds, s);
@@ -1944,10 +1913,9 @@ int svx_run_batch(void *const *states, const int64_t *handles, const int32_t *pa
                 return fail(SVX_E_ARG, "batch states must be non-null and 16-byte aligned");
             const LaunchInfo &li = pr->launches[passes[i]];
             if (li.empty) continue;
-            if (li.small || li.dma)
+            if (li.small)
                 return fail(SVX_E_ARG, "batch item " + std::to_string(i) +
-                                           ": only passes of the single-buffer kernels batch (state >= 2^10 amplitudes, "
-                                           "SVX_KERNEL_AUTO or SVX_KERNEL_SINGLE)");
+                                           ": only passes of the single-buffer kernels batch (state >= 2^10 amplitudes)");
             unsigned char *base = static_cast<unsigned char *>(pr->dev);
             BatchItem it{};
             it.st = states[i];
@@ -2017,7 +1985,6 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap) {
     std::string k;
     if (li.empty) k = "none";
     else if (li.small) k = std::string("k_pass_small<") + T + ">";
-    else if (li.dma) k = "k_pass_dma<" + std::to_string(li.K) + (li.dense ? ",dense>" : ">");
     else if (li.d4) k = "k_pass_d4<" + std::to_string(li.K) + ">";
     else if (li.mux) k = "k_mux<" + std::to_string(li.K) + "," + std::to_string(mux_register_bits(li.K)) + ">";
     else k = std::string("k_pass<") + T + "," + std::to_string(li.K) + (li.dense ? ",true>" : ",false>");

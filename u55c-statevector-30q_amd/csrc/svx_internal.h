@@ -56,7 +56,6 @@ struct PassDev {
                                  // (dense groups of type 4 rely on it); 0: no tile scaling
     int32_t torder;              // workgroup -> tile order: 0 identity, -1 bit reversal, -2 XCD-contiguous,
                                  // r > 0 rotate left by r
-    uint64_t dma_u[64];          // DMA kernel: element offset of tile elements 128 j .. from bits >= 7
 };
 
 struct GroupDev {
@@ -146,9 +145,6 @@ hipError_t launch_mux_queue(int K, const QueueJob *jobs, QueueCtl *ctl, uint32_t
                             uint64_t idle_ticks, uint32_t grid, size_t lds, hipStream_t s);
 // double-buffered complex64 pass kernel (one QSV2 tile of 2^K per LDS buffer, K in {12, 13});
 // register groups of 4 tile bits
-size_t pass_dma_lds_bytes(int K, int m);
-hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
-                           const float *wd, uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s);
 size_t pass_lds_bytes(int K, int m, int k, int elem_bytes, bool small, int nbits);
 hipError_t launch_permute(int dtype, void *dst, const void *src, const PermDev *pd, uint32_t grid,
                           size_t lds, hipStream_t s);

@@ -840,167 +840,6 @@ __global__ __launch_bounds__(kNT, (DN && K < 14) ? 2 : 1) void k_pass_batch(cons
                             reinterpret_cast<const OpDev<float> *>(it.od), it.wd, tile_order(b - it.tile0, it.pd), smem);
 }
 
-// ------------------------------------------------------------------------------------------ pass, DMA
-// Persistent, double-buffered pass kernel for complex64 with one QSV2 tile per LDS buffer (K = k).
-// Wave 0 is the loader: it streams the next tile straight into the idle LDS buffer with 16-byte
-// global_load_lds while waves 1..4 (the workers) apply the pass's groups to the current buffer and
-// store it back; the loader waits for its own loads only, the workers never wait on their stores,
-// so HBM reads, the gate math and HBM writes of consecutive tiles overlap.  The two roles run
-// separate loops with the same barrier count per tile.
-//
-// LDS layout: 16-byte slots, logical pair P (elements 2P, 2P+1) in slot P ^ ((P >> 4) & 15); the
-// byte address of element e is swz(e), an XOR-linear map, so swz(base | off) = swz(base) ^ swz(off)
-// for the disjoint bit sets of a register group.
-typedef __attribute__((address_space(3))) void lds_void_t;
-
-__device__ __forceinline__ void lds_barrier() {
-    // LDS writes of this wave complete, then the workgroup barrier; global stores stay in flight
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-
-template <int K, int R, int NW, bool DN>
-__global__ __launch_bounds__(64 + NW) void k_pass_dma(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
-                                                   const GroupDev *__restrict__ gd,
-                                                   const OpDev<float> *__restrict__ od, const float *__restrict__ wd,
-                                                   uint64_t ntiles) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    constexpr uint32_t NE = 1u << K, NS = NE / 2, BUF = NE * 8;
-    constexpr uint32_t kNTD = 64 + NW;       // loader wave + NW worker threads
-    static_assert((NE >> R) % NW == 0, "every worker owns the same number of register slots");
-    constexpr int NR = 1 << R;
-    constexpr uint32_t NSLOT = NE >> R;
-    uint64_t *sbo = reinterpret_cast<uint64_t *>(smem + 2 * BUF);
-    const int m = pd->m, lowb = K - m;
-    const uint32_t lowmask = (1u << lowb) - 1u;
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t q = tid; q < (1u << m); q += kNTD) {
-        uint64_t o = 0;
-        for (int i = 0; i < m; ++i)
-            if ((q >> i) & 1u) o |= uint64_t(1) << pd->h[i];
-        sbo[q] = o;
-    }
-    uint64_t t = blockIdx.x;
-    const uint64_t G = gridDim.x;
-    if (t >= ntiles) return;
-    auto tile_base = [&](uint64_t tt) {
-        uint64_t x = tt << lowb;
-        for (int i = 0; i < m; ++i) x = ins0_64(x, pd->hs[i]);
-        return x;
-    };
-    const int ng = pd->ngroups, g0 = pd->group0;
-    lds_barrier();
-    if (tid < 64) {
-        // ------------------------------------------------ loader wave
-        const uint32_t lane = tid;
-        // DMA instruction j moves slots S = 64 j + lane, i.e. logical pairs P = 64 j + (lane ^ h) with
-        // h = (4 (j & 3) + lane / 16) & 15: elements 128 j + eo, eo < 128.  The element offset inside
-        // the tile is U(j) | V(lane, j & 3): U(j) (uniform, from bits >= 7) and V (bits < 7, per lane).
-        uint64_t vlo[4];
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t h = (4u * q + lane / 16u) & 15u;
-            const uint32_t eo = 2u * (lane ^ h);
-            vlo[q] = (lowb < 7 ? sbo[eo >> lowb] : 0) | (eo & lowmask);
-        }
-        const unsigned char *sbytes = reinterpret_cast<const unsigned char *>(st);
-        auto dma_tile = [&](unsigned char *buf, uint64_t tt) {
-            const unsigned char *tile = sbytes + tile_base(tt) * sizeof(C2<float>);
-#pragma unroll 16
-            for (uint32_t j = 0; j < NS / 64; ++j)
-                __builtin_amdgcn_global_load_lds(tile + (pd->dma_u[j] | vlo[j & 3u]) * sizeof(C2<float>),
-                                                 (lds_void_t *)(buf + j * 1024u), 16, 0, 0);
-        };
-        int cur = 0;
-        dma_tile(smem, t);
-        for (; t < ntiles; t += G) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            lds_barrier();
-            if (t + G < ntiles) dma_tile(smem + (cur ^ 1) * BUF, t + G);
-            for (int g = 0; g < ng; ++g) lds_barrier();
-            cur ^= 1;
-        }
-    } else {
-        // ------------------------------------------------ worker waves
-        const uint32_t w = tid - 64;
-        int cur = 0;
-        for (; t < ntiles; t += G) {
-            lds_barrier();
-            unsigned char *buf = smem + cur * BUF;
-            for (int g = 0; g < ng; ++g) {
-                const GroupDev &Gd = gd[g0 + g];
-                if constexpr (DN) {
-                    if (Gd.type != 0) {
-                        dense_group<K, NW / 64, true>(buf, Gd, wd, w);
-                        lds_barrier();
-                        continue;
-                    }
-                }
-                uint32_t boff[NR];
-                boff[0] = 0;
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const uint32_t bq = swz(1u << Gd.b[q]);
-#pragma unroll
-                    for (int r = (1 << q); r < (2 << q); ++r) boff[r] = boff[r - (1 << q)] ^ bq;
-                }
-                const int nops = Gd.nops, o0 = Gd.op0;
-#pragma unroll 1
-                for (uint32_t sl = w; sl < NSLOT; sl += NW) {
-                    uint32_t bidx = sl;
-#pragma unroll
-                    for (int q = 0; q < R; ++q) bidx = ins0_32(bidx, Gd.bs[q]);
-                    const uint32_t bb = swz(bidx);
-                    C2<float> a[NR];
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) a[r] = *reinterpret_cast<const C2<float> *>(buf + (bb ^ boff[r]));
-#pragma unroll 1
-                    for (int o = 0; o < nops; ++o) {
-                        const OpDev<float> &op = od[o0 + o];
-                        const bool ts = (bidx & op.cmt) == op.cmt;
-                        apply_op<float, R>(a, op, ts);
-                    }
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) *reinterpret_cast<C2<float> *>(buf + (bb ^ boff[r])) = a[r];
-                }
-                lds_barrier();
-            }
-            const uint64_t tb = tile_base(t);
-            float *base = reinterpret_cast<float *>(st);
-#pragma unroll 4
-            for (uint32_t S = w; S < NS; S += NW) {
-                const uint32_t e = 2u * (S ^ ((S >> 4) & 15u));
-                const uint64_t gi = tb | sbo[e >> lowb] | (e & lowmask);
-                const float4 v = *reinterpret_cast<const float4 *>(buf + 16u * S);
-                st_stream(base + 2 * gi, v);
-            }
-            cur ^= 1;
-        }
-    }
-}
-
-size_t pass_dma_lds_bytes(int K, int m) {
-    return (size_t(2) << K) * 8 + (size_t(1) << m) * sizeof(uint64_t);
-}
-
-hipError_t launch_pass_dma(int K, void *st, const PassDev *pd, const GroupDev *gd, const OpDev<float> *od,
-                           const float *wd, uint64_t ntiles, uint32_t grid, size_t lds, hipStream_t s) {
-    C2<float> *p = reinterpret_cast<C2<float> *>(st);
-    if (wd) {   // pass with dense groups
-        switch (K) {
-        case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, true>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
-        case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512, true>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, wd, ntiles); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    }
-    switch (K) {
-    case 12: hipLaunchKernelGGL((k_pass_dma<12, 4, 256, false>), dim3(grid), dim3(64 + 256), lds, s, p, pd, gd, od, wd, ntiles); break;
-    case 13: hipLaunchKernelGGL((k_pass_dma<13, 4, 512, false>), dim3(grid), dim3(64 + 512), lds, s, p, pd, gd, od, wd, ntiles); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
 
 // One workgroup runs a whole small state (<= 2^kSmallMaxBits amplitudes), op by op, directly
 // from the MUX2 definition.  Also the independent cross-check of k_pass in the tests.
@@ -1558,8 +1397,6 @@ hipError_t set_kernel_attributes() {
     SVX_ATTR((k_pass_batch<14, true>))
     SVX_ATTR((k_pass<double, 10, false>)) SVX_ATTR((k_pass<double, 11, false>)) SVX_ATTR((k_pass<double, 12, false>))
     SVX_ATTR((k_pass<double, 13, false>)) SVX_ATTR((k_permute<float>)) SVX_ATTR((k_permute<double>))
-    SVX_ATTR((k_pass_dma<12, 4, 256, false>)) SVX_ATTR((k_pass_dma<13, 4, 512, false>))
-    SVX_ATTR((k_pass_dma<12, 4, 256, true>)) SVX_ATTR((k_pass_dma<13, 4, 512, true>))
     if (e == hipSuccess) e = set_mux_attributes();
 #undef SVX_ATTR
     return e;

@@ -30,7 +30,7 @@ SYMBOLS = ["svx_init", "svx_run_segment", "svx_run_segment_ex", "svx_engine", "s
 
 FUSE = {"auto": 0, "on": 0, "off": 1, "dense4": 2}
 DENSE_MATH = {"f32": 0, "split16": 1}
-KERNEL = {"auto": 0, "single": 1, "dma": 2}
+KERNEL = {"auto": 0, "single": 1}
 TILE_ORDER = {"tunable": 0, "identity": 1, "xcd": 2}
 MUX_ENGINE = {"auto": 0, "valu": 1}
 
