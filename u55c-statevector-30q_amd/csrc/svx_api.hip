@@ -424,6 +424,45 @@ struct DenseUnit {
     uint32_t dmask = 0;
 };
 
+// The four tile bits that index the 16 columns of a 4-qubit unit's column block (lane l % 16 of a
+// wavefront step).  A wavefront's 8-byte LDS access is served 16 lanes at a time, and the padded
+// layout (element e at 8 (e + (e >> 5)) bytes) keeps those 16 lanes free of bank conflicts when their
+// padded element offsets differ modulo 32 (64 banks of 4 bytes; measured on gfx950 over every 4-bit
+// unit set and lane-bit choice of a 2^13 tile, this rule never passed a conflicting pattern).  The
+// lowest four free bits conflict for some unit sets (for instance units on bits {3, 4, 5, 6}: columns
+// on bits 0, 1, 2, 7); then the first conflict-free choice of four free bits is taken.
+static uint32_t pad_elem(uint32_t e) { return e + (e >> 5); }
+static bool lanes_conflict_free(uint32_t jmask) {
+    uint32_t seen = 0;
+    for (uint32_t j = 0; j < 16; ++j) {
+        uint32_t e = 0, m = jmask;
+        for (int i = 0; i < 4; ++i) {
+            const int b = __builtin_ctz(m);
+            m &= m - 1;
+            e |= ((j >> i) & 1u) << b;
+        }
+        const uint32_t bank = pad_elem(e) & 31u;
+        if ((seen >> bank) & 1u) return false;
+        seen |= 1u << bank;
+    }
+    return true;
+}
+static uint32_t unit_lane_bits(int K, uint32_t unit_mask) {
+    int fb[32], nf = 0;
+    for (int b = 0; b < K; ++b)
+        if (!((unit_mask >> b) & 1u)) fb[nf++] = b;
+    const uint32_t lowest = (1u << fb[0]) | (1u << fb[1]) | (1u << fb[2]) | (1u << fb[3]);
+    if (lanes_conflict_free(lowest)) return lowest;
+    for (int a = 0; a < nf; ++a)
+        for (int b = a + 1; b < nf; ++b)
+            for (int c = b + 1; c < nf; ++c)
+                for (int d = c + 1; d < nf; ++d) {
+                    const uint32_t m = (1u << fb[a]) | (1u << fb[b]) | (1u << fb[c]) | (1u << fb[d]);
+                    if (lanes_conflict_free(m)) return m;
+                }
+    return lowest;
+}
+
 // VALU cost of one op per amplitude, in FMAs (a general 2x2 MUX2 = 8), by OpKind
 const double kOpCost[] = {8, 4, 1, 8, 4, 2, 2, 8, 2, 2};
 // smallest MUX2 cost of a run of diagonal ops that becomes a table (one complex multiply and one
@@ -1233,6 +1272,7 @@ int build_prepared(const RawSeg &seg, int nbits, int dtype, const Opts &opt, Pre
                     // FP32, three real products (Gauss): wd[(s 4 + t) 64 + l] = W_s[l % 16][4 (l / 16) + t],
                     // W_0 = Ur + Ui, W_1 = -Ui, W_2 = Ur (A operands of v_mfma_f32_16x16x4f32)
                     gd.type = 5;
+                    gd.jmask = (int32_t)unit_lane_bits(K, (1u << qs[0]) | (1u << qs[1]) | (1u << qs[2]) | (1u << qs[3]));
                     for (int sidx = 0; sidx < 3; ++sidx)
                         for (int t = 0; t < 4; ++t)
                             for (int l = 0; l < 64; ++l) {

@@ -276,10 +276,11 @@ typedef float f4acc __attribute__((ext_vector_type(4)));
 // 4 g .. 4 g + 3 of column l % 16 and writes the same four outputs (no other lane touches them).
 // Operands: wd[(s 4 + t) 64 + l] = W_s[l % 16][4 (l / 16) + t] with W_0 = Wr + Wi, W_1 = -Wi, W_2 = Wr.
 // Each wavefront step carries up to two 16-column blocks, independent MFMA chains.
-// Addresses: column c = 16 blk + j of the tile deposits its bits on the tile bits outside the unit's
-// four (ascending).  The deposit of disjoint parts is the sum (padded layout) or XOR (swizzled) of
-// their deposits, so a lane's part (j, and its components' offsets) is formed once per unit and the
-// block part (wavefront-uniform) in scalar registers per block.
+// Addresses: column j of block blk sits on the tile element with j's four bits on the unit's lane
+// bits (GroupDev::jmask, chosen by the host so that 16 lanes hit distinct LDS banks) and blk's bits
+// on the remaining free bits (ascending).  The deposit of disjoint parts is the sum (padded layout)
+// or XOR (swizzled) of their deposits, so a lane's part (j, and its components' offsets) is formed
+// once per unit and the block part (wavefront-uniform) in scalar registers per block.
 // the lane's twelve A-operand values of a 4-qubit three-product unit (loaded ahead of the unit by
 // callers that can, so the loads overlap a barrier or the tile load)
 struct Gauss4W {
@@ -301,14 +302,36 @@ __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev 
         w1[t] = gw.w[4 + t];
         w2[t] = gw.w[8 + t];
     }
-    auto dep = [&](uint32_t c) {
+    // lane bits and the block bits' positions (wavefront-uniform)
+    const uint32_t jm = __builtin_amdgcn_readfirstlane((uint32_t)G.jmask);
+    uint32_t rest = ((1u << K) - 1u) & ~jm;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) c = ins0_32(c, G.dqs[i]);
-        return c;
+    for (int i = 0; i < 4; ++i) rest &= ~(1u << G.dqs[i]);
+    constexpr int NBB = K - 8;   // block-index bits
+    uint32_t bp[NBB > 0 ? NBB : 1];
+#pragma unroll
+    for (int i = 0; i < NBB; ++i) {
+        bp[i] = __builtin_ctz(rest);
+        rest &= rest - 1u;
+    }
+    auto dep = [&](uint32_t blk) {
+        uint32_t e = 0;
+#pragma unroll
+        for (int i = 0; i < NBB; ++i) e |= ((blk >> i) & 1u) << bp[i];
+        return e;
     };
+    uint32_t ej = 0;
+    {
+        uint32_t m = jm;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ej |= ((j >> i) & 1u) << __builtin_ctz(m);
+            m &= m - 1u;
+        }
+    }
     ColOff<4, SWZ> coff;
     coff.init(G);
-    const uint32_t lane_off = TileAddr<SWZ>::of(dep(j));
+    const uint32_t lane_off = TileAddr<SWZ>::of(ej);
     uint32_t co[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) co[t] = TileAddr<SWZ>::comb(lane_off, coff(4u * g + t));
@@ -324,7 +347,7 @@ __device__ __forceinline__ void dense_gauss4(unsigned char *buf, const GroupDev 
         for (uint32_t h = 0; h < BPS; ++h) {
             const uint32_t blk = b + h * NWAVE;
             ok[h] = blk < NB;   // uniform
-            ub[h] = TileAddr<SWZ>::of(dep((ok[h] ? blk : b) * 16u));
+            ub[h] = TileAddr<SWZ>::of(dep(ok[h] ? blk : b));
 #pragma unroll
             for (int t = 0; t < 4; ++t) x[h][t] = ldc(buf, TileAddr<SWZ>::comb(co[t], ub[h]));
         }

@@ -74,6 +74,8 @@ struct GroupDev {
     int32_t dqs[5];              // dq sorted ascending (column deposit)
     int32_t wexp;                // split-FP16 dense: the matrix was scaled by 2^wexp before the split
     int32_t exact;               // dense: entries all in {0, +-1, +-i} (FP32 products exact; statistics)
+    int32_t jmask;               // type 5: the four tile bits that index the 16 columns of a column
+                                 // block (lane l % 16); the block index runs over the other free bits
     int64_t w_off;               // dense: float offset of the matrix (type 1: MFMA A-operand order;
                                  // type 2: row-major re, im)
 };

@@ -445,9 +445,13 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
     move_tile<float, K, false, NT>(st, tile, sbo, tb, k, lowb, tid);
 }
 
-// K = 13: two workgroups per CU (two 64 KiB tiles in LDS), so at most 128 VGPRs
+// Occupancy: the second __launch_bounds__ argument is the minimum number of wavefronts per SIMD.
+// R = 4 (K <= 13): four per SIMD, so at most 128 VGPRs -- at K = 13 two eight-wavefront
+// workgroups per CU (LDS holds two 64 KiB tiles), at smaller K more and smaller ones.  K = 14
+// (R = 5): one workgroup per CU (one 128 KiB tile), two wavefronts per SIMD.  The batched and
+// queued forms keep the compiler's choice: held to 128 VGPRs they spill.
 template <int K, int R, bool DN>
-__global__ __launch_bounds__(1 << (K - R), K == 13 ? 2 : 1) void k_mux(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
+__global__ __launch_bounds__(1 << (K - R), R == 4 ? 4 : 2) void k_mux(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                                       const GroupDev *__restrict__ gd,
                                                       const OpDev<float> *__restrict__ od, const float *__restrict__ wd) {
     extern __shared__ __align__(16) unsigned char smem[];
