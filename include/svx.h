@@ -162,7 +162,8 @@ int svx_prepared_kernel(int64_t handle, int32_t pass, char *name, int32_t cap);
  * out[0] MUX2 register groups, out[1] MUX2 ops, out[2] dense units on 4 qubits, out[3] dense units on
  * 5 qubits, out[4] dense units whose entries are all 0, +-1 or +-i (FP32 products exact, so
  * permutations stay bit-exact), out[5] MUX2 ops applied on the FP32 matrix cores (the MUX2 kernel
- * alternates general 2x2 ops between the vector units and v_mfma_f32_4x4x1), out[6] diagonal tables
+ * runs general 2x2 ops on v_mfma_f32_4x4x1_16b_f32 unless svx_options.mux_engine is SVX_MUX_VALU),
+ * out[6] diagonal tables
  * (runs of diagonal ops applied as one table of phases), out[7] the workgroup-to-tile order of the
  * pass (-2: each XCD takes a contiguous range of tiles, 0: identity; see svx_tune_prepared)
  * (out has 8 entries).  No reference counterpart. */
