@@ -448,8 +448,10 @@ __device__ __forceinline__ void mux_tile(C2<float> *__restrict__ st, const PassD
 // Occupancy: the second __launch_bounds__ argument is the minimum number of wavefronts per SIMD.
 // R = 4 (K <= 13): four per SIMD, so at most 128 VGPRs -- at K = 13 two eight-wavefront
 // workgroups per CU (LDS holds two 64 KiB tiles), at smaller K more and smaller ones.  K = 14
-// (R = 5): one workgroup per CU (one 128 KiB tile), two wavefronts per SIMD.  The batched and
-// queued forms keep the compiler's choice: held to 128 VGPRs they spill.
+// (R = 5): one workgroup per CU (one 128 KiB tile), two wavefronts per SIMD.  The batched form
+// reads its descriptors through the constant address space and fits the same bound; the queued
+// form keeps the compiler's choice (its descriptors are read with vector loads, and held to 128
+// VGPRs it spills).
 template <int K, int R, bool DN>
 __global__ __launch_bounds__(1 << (K - R), R == 4 ? 4 : 2) void k_mux(C2<float> *__restrict__ st, const PassDev *__restrict__ pd,
                                                       const GroupDev *__restrict__ gd,
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(1 << (K - R), R == 4 ? 4 : 2) void k_mux(C2<float> 
 }
 
 template <int K, int R, bool DN>
-__global__ __launch_bounds__(1 << (K - R)) void k_mux_batch(const BatchItem *__restrict__ items, int n_items) {
+__global__ __launch_bounds__(1 << (K - R), R == 4 ? 4 : 2) void k_mux_batch(const BatchItem *__restrict__ items, int n_items) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint64_t b = blockIdx.x;
     int lo = 0, hi = n_items - 1;
@@ -477,10 +479,19 @@ __global__ __launch_bounds__(1 << (K - R)) void k_mux_batch(const BatchItem *__r
                            ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32);
         return reinterpret_cast<decltype(ptr)>(u);
     };
-    const PassDev *pd = uni(it.pd);
+    // the descriptors are read-only for the launch: read them through the constant address space,
+    // so that the compiler keeps uniform descriptor words in scalar registers as it does for k_mux's
+    // kernel arguments
+    auto cst = [](auto *ptr) {
+        using T = std::remove_cv_t<std::remove_pointer_t<decltype(ptr)>>;
+        const __attribute__((address_space(4))) T *c =
+            (const __attribute__((address_space(4))) T *)reinterpret_cast<uintptr_t>(ptr);
+        return (const T *)c;
+    };
+    const PassDev *pd = cst(uni(it.pd));
     const uint64_t t0 = uni(reinterpret_cast<const char *>(it.tile0)) - static_cast<const char *>(nullptr);
-    mux_tile<K, R, DN>(reinterpret_cast<C2<float> *>(uni(it.st)), pd, uni(it.gd),
-                   reinterpret_cast<const OpDev<float> *>(uni(it.od)), uni(it.wd), tile_order(b - t0, pd), smem);
+    mux_tile<K, R, DN>(reinterpret_cast<C2<float> *>(uni(it.st)), pd, cst(uni(it.gd)),
+                   cst(reinterpret_cast<const OpDev<float> *>(uni(it.od))), cst(uni(it.wd)), tile_order(b - t0, pd), smem);
 }
 
 // Persistent queue of whole-state circuits (state = one 2^K tile, one pass each).  The host
