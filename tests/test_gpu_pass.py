@@ -297,6 +297,31 @@ def _gain_segment(rng, L, k, n_passes, n_ops, gain, max_m=6):
     return (np.asarray(w, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
 
 
+# unit bit sets whose lowest four free tile bits put two lanes of a 16-lane LDS access on one bank
+# (the library then maps the 16 columns of a block to other free bits); listed with conflict-free
+# sets in the same pass so both column mappings meet in one tile
+_CONFLICTING_UNITS = [(3, 4, 5, 6), (2, 3, 4, 7), (1, 3, 5, 6), (0, 3, 4, 5), (2, 4, 6, 8), (1, 4, 5, 6),
+                      (0, 1, 4, 5), (2, 3, 6, 7)]
+
+
+@pytest.mark.parametrize("n,k,hi", [(20, 13, [16, 17, 18, 19]), (22, 13, [])])
+def test_dense_units_with_remapped_columns(gpu, n, k, hi):
+    """Fused-form dense records on tile-bit sets for which the library moves the column lane bits
+    off the lowest free bits (to keep LDS reads free of bank conflicts), interleaved with units on
+    ordinary sets: the complex64 result against the complex128 golden model, to the FP32 bound."""
+    rng = np.random.default_rng(n + len(hi))
+    units = []
+    for i, u in enumerate(_CONFLICTING_UNITS):
+        units.append(tuple(int(b) for b in rng.permutation(u)))   # matrix bit order need not be ascending
+        if i % 2:
+            units.append(tuple(int(b) for b in rng.choice(13, 4, replace=False)))
+    seg = _dense_pass_words(rng, n, k, hi, units)
+    s0 = _rand_state(rng, n)
+    ref = model.run_segment(s0.copy(), 1, seg, np.complex128)
+    got = run_hip(s0.astype(np.complex64), seg)
+    _check_c64(ref, got, units, seg)
+
+
 @pytest.mark.parametrize("gain,scale", [(1.0, 1.0), (1.0, 1e6), (1.0, 1e-25), (1.15, 1.0), (2.0, 1.0), (0.5, 1e3)])
 @pytest.mark.parametrize("math", ["f32", "split16_tile", "split16_col"])
 def test_pass_fusion_tile_scaling(gpu, gain, scale, math):
