@@ -226,8 +226,9 @@ int svx_product_state(void *state, int32_t n, int32_t dtype, const double *amps,
 int svx_reduce_f64(const void *state, int32_t n, int32_t dtype, const int32_t *fmap, int32_t n_low,
                    double *out_norm, double *out_marginals, void *stream);
 
-/* svx_reduce_f64 without allocation, for callers that cannot allocate device memory (code under HIP
- * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+/* svx_reduce_f64 without allocation, for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc; it synchronises `stream`, so it is not for use while the
+ * stream is being captured into a graph).  No counterpart in the
  * reference.  Same arguments, kernels, reduction order and results as svx_reduce_f64; the FP64
  * partials, the bin configuration and the bins live in a library-owned static device array of
  * 4 MiB + 6 KiB per device (held until the process ends), so n_low is limited to 8 (SVX_E_CAPACITY
