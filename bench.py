@@ -428,6 +428,11 @@ def main():
                     help="for rocprofv3 kernel traces: the 30-qubit controlled-phase ladder's complex64 passes "
                          "(tile orders tuned first), once, then the bit reversal twice out of place and once in "
                          "place; prints per-pass work")
+    ap.add_argument("--fused-probe", type=int, default=0, metavar="N",
+                    help="for rocprofv3 kernel traces: the N-qubit fused-form workload (depth 40, seed 2), "
+                         "complex64, once as plain MUX2 records (k = 13, fusion off) and once in the fused "
+                         "form the autotuner picks at 32 and 34 qubits (k = 13, 4-qubit records); prints the "
+                         "pass counts and the per-pass kernels")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: an 8 GiB complex64 copy (svx_convert, calibration) twice, "
                          "the tile-order tuning launches (on the copy's destination), then one step of the "
@@ -474,6 +479,30 @@ def main():
         work = [dict(libsvx.prepared_pass_stats(h, 0), kernel=libsvx.prepared_kernel(h, 0)) for h in hs]
         print(json.dumps({"passes": len(hs), "per_pass": work,
                           "in_place_sweeps": len(libsvx.permute_plan(list(range(n))[::-1]))}))
+        return
+    if a.fused_probe:
+        nf = a.fused_probe
+        qf = circuit.workload_fused(nf)
+        sf = torch.empty(1 << nf, dtype=torch.complex64, device="cuda:0")
+        res = {}
+        for form in ("plain", "w4"):
+            prog, fmap, st = runtime.compile_form(qf, 13, form)
+            with libsvx.options(fuse=runtime.form_fusion(form)):
+                hs = [libsvx.prepare(w, libsvx.C64, nf) for _, w in prog]
+            libsvx.zero_state(sf)
+            for h in hs:
+                libsvx.run_prepared(sf, h)
+            torch.cuda.synchronize()
+            kinds = {}
+            for h in hs:
+                for p in range(libsvx.prepared_info(h)[1]):
+                    kn = libsvx.prepared_kernel(h, p)
+                    kinds[kn] = kinds.get(kn, 0) + 1
+            res[form] = {"passes": st.n_passes, "launches": sum(kinds.values()), "kernels": kinds,
+                         "norm_f64": libsvx.reduce_f64(sf, [])[0]}
+            for h in hs:
+                libsvx.release_prepared(h)
+        print(json.dumps({"fused_probe": nf, **res}))
         return
     if a.traffic_probe:
         prog, fmap, st = compiler.compile_gpu(qc)
