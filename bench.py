@@ -430,9 +430,9 @@ def main():
                          "place; prints per-pass work")
     ap.add_argument("--fused-probe", type=int, default=0, metavar="N",
                     help="for rocprofv3 kernel traces: the N-qubit fused-form workload (depth 40, seed 2), "
-                         "complex64, once as plain MUX2 records (k = 13, fusion off) and once in the fused "
-                         "form the autotuner picks at 32 and 34 qubits (k = 13, 4-qubit records); prints the "
-                         "pass counts and the per-pass kernels")
+                         "complex64, run once in each of three forms at k = 13: plain MUX2 records (fusion "
+                         "off), MUX2 records with library fusion (the form the autotuner picks at 32 and 34 "
+                         "qubits) and fused 4-qubit records; prints the pass counts and the per-pass kernels")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: an 8 GiB complex64 copy (svx_convert, calibration) twice, "
                          "the tile-order tuning launches (on the copy's destination), then one step of the "
@@ -485,7 +485,7 @@ def main():
         qf = circuit.workload_fused(nf)
         sf = torch.empty(1 << nf, dtype=torch.complex64, device="cuda:0")
         res = {}
-        for form in ("plain", "w4"):
+        for form in ("plain", "lib", "w4"):
             prog, fmap, st = runtime.compile_form(qf, 13, form)
             with libsvx.options(fuse=runtime.form_fusion(form)):
                 hs = [libsvx.prepare(w, libsvx.C64, nf) for _, w in prog]
@@ -505,7 +505,7 @@ def main():
         print(json.dumps({"fused_probe": nf, **res}))
         return
     if a.traffic_probe:
-        prog, fmap, st = compiler.compile_gpu(qc)
+        prog, fmap, st = compiler.compile_gpu(qc, cluster=0 if a.plain else compiler.GPU_CLUSTER)
         with libsvx.options(**(dict(fuse="off") if a.plain else {})):
             hs = [libsvx.prepare(w, libsvx.C64, n) for _, w in prog]
         src = torch.empty(1 << n, dtype=torch.complex64, device="cuda:0")

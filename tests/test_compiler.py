@@ -126,10 +126,11 @@ def test_compile_gpu_streams_vs_dense(n):
     for qc in [zoo(n, 5), C.make("qft", n), C.make("rqc", n), C.u3_cx_layers(n, 6, 4), C.cp_ladder(n)]:
         ref = dense.simulate(qc.tuples(), n)
         for k in (4, 5, 6):
-            for sched in ("window", "inorder"):
-                prog, fmap, st = K.compile_gpu(qc, k=k, scheduler=sched)
+            # window scheduling cluster by cluster (the default) and op by op, and in-order scheduling
+            for sched, cluster in (("window", K.GPU_CLUSTER), ("window", 0), ("inorder", 0)):
+                prog, fmap, st = K.compile_gpu(qc, k=k, scheduler=sched, cluster=cluster)
                 out = model.run_program(prog, n, 1, fmap, ctype=np.complex128)
-                assert 1 - judge.fidelity(ref, out) < 1e-10, (qc.name, k, sched)
+                assert 1 - judge.fidelity(ref, out) < 1e-10, (qc.name, k, sched, cluster)
 
 
 @pytest.mark.parametrize("n", [6, 9])
@@ -219,3 +220,64 @@ def test_workloads_are_seeded_and_shaped():
     ops, _ = K.circuit_to_ops(lad, st)
     assert len(ops) == 465
     assert math.isclose(lad.data[1].params[0], math.pi / 2)
+
+
+def test_schedule_clusters_keeps_dense_blocks_whole():
+    """Every block that schedule_clusters takes as a dense unit lands in one pass, its ops in
+    program order; the passes hold every op once, and per-qubit op order is preserved."""
+    qc = C.u3_cx_layers(16, 10, 5)
+    ops, _ = K.circuit_to_ops(qc, K.Stats(), True, max_width=10)
+    k, M = 10, 4
+    passes = K.schedule_clusters(ops, 16, k, K.Stats(), M, 4)
+    where = {id(op): (i, j) for i, p in enumerate(passes) for j, (op, _) in enumerate(p)}
+    assert sorted(where) == sorted(id(op) for op in ops)
+    dense_blocks = [b for b in K.fuse_ops(ops, 4)
+                    if len(b.qubits) >= 2 and sum(K.op_cost(o) for o in b.ops) > K.dense_cost(4)]
+    assert dense_blocks
+    for b in dense_blocks:
+        pos = [where[id(o)] for o in b.ops]
+        assert len({i for i, _ in pos}) == 1 and pos == sorted(pos)
+    for q in range(16):   # ops touching qubit q keep their program order
+        seq = [where[id(op)] for op in ops if q in op.q]
+        assert seq == sorted(seq)
+    for p in passes:
+        assert K.pass_window({x for _, ph in p for x in ph}, k, M) is not None
+
+
+def test_fuse_ops_terminates_on_ops_wider_than_the_block():
+    """A multi-controlled X on 6 qubits (Grover's diffusion) is wider than a 4-qubit block: it
+    forms a block of its own and clustering still covers every op once, in dependency order."""
+    qc = C.make("grover", 6)
+    ops, _ = K.circuit_to_ops(qc, K.Stats(), True, max_width=13)
+    assert max(len(op.q) for op in ops) > 4
+    blocks = K.fuse_ops(ops, 4)
+    flat = [id(o) for b in blocks for o in b.ops]
+    assert sorted(flat) == sorted(id(o) for o in ops)
+    for b in blocks:
+        assert b.ops and (len(b.qubits) <= 4 or len(b.ops) == 1)
+    for n in (6, 8):
+        qc = C.make("grover", n)
+        ref = dense.simulate(qc.tuples(), n)
+        for prog, fmap, _ in (K.compile_gpu(qc, k=6), K.compile_fused(qc, k=6, width=4)):
+            out = model.run_program(prog, n, 1, fmap, ctype=np.complex128)
+            assert 1 - judge.fidelity(ref, out) < 1e-10
+
+
+def test_compile_gpu_takes_clustered_schedule_by_pass_count():
+    """From CLUSTER_MIN_QUBITS on, the default schedule is the clustered one exactly when it adds at
+    most CLUSTER_MAX_EXTRA_PASSES of passes; below, passes are drawn op by op.  Results match the
+    golden model run of the op-by-op stream either way."""
+    seen = set()
+    for qc in [C.u3_cx_layers(22, 4, 3), C.make("vqe", 22), C.make("rqc", 22), C.make("qft", 22),
+               C.u3_cx_layers(21, 4, 3)]:
+        n = qc.num_qubits
+        k = 13
+        M = K.GPU_MAX_HIGH
+        ops, _ = K.circuit_to_ops(qc, K.Stats(), True, max_width=k)
+        by_op = K.schedule_window(ops, n, k, K.Stats(), M)
+        by_cl = K.schedule_clusters(ops, n, k, K.Stats(), M, K.GPU_CLUSTER)
+        want = n >= K.CLUSTER_MIN_QUBITS and len(by_cl) <= len(by_op) * (1 + K.CLUSTER_MAX_EXTRA_PASSES)
+        prog, fmap, st = K.compile_gpu(qc, k=k)
+        assert st.clustered == want and st.n_passes == len(by_cl if want else by_op), qc.name
+        seen.add(want)
+    assert seen == {True, False}

@@ -21,3 +21,6 @@ GPU_K = 13
 GPU_MIN_SUB = 6
 GPU_MAX_HIGH = GPU_K - GPU_MIN_SUB
 GPU_MAX_OPS = 64
+# Width of the op clusters the MI355X scheduler keeps inside one pass: a 4-qubit cluster is one
+# dense unit of the matrix-core engine (16x16 complex, three real 16x16x4 products).
+GPU_CLUSTER = 4

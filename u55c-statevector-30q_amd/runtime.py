@@ -67,10 +67,13 @@ TUNE_KS = (12, 13, 14)
 
 
 def compile_form(qc, k, form, max_high=None):
-    """Compile for one execution form: "plain" (MUX2 records, every op on the vector units),
-    "lib" (MUX2 records, the library clusters each pass's ops into dense units), "w4" / "w5"
+    """Compile for one execution form: "plain" (MUX2 records scheduled op by op, every op on the
+    vector units), "lib" (MUX2 records scheduled cluster by cluster; the library fuses each pass's
+    clusters into dense units), "w4" / "w5"
     (fused-form records: the compiler emits dense 4- / 5-qubit units)."""
-    if form in ("plain", "lib"):
+    if form == "plain":
+        return compile_gpu(qc, k=k, max_high=max_high, cluster=0)
+    if form == "lib":
         return compile_gpu(qc, k=k, max_high=max_high)
     if form in ("w4", "w5"):
         return compile_fused(qc, k=k, width=int(form[1]), max_high=max_high)
@@ -301,7 +304,8 @@ class SVRuntime:
         if form in ("w4", "w5"):
             prog, fmap, stats = compile_fused(qc, k=kk, width=int(form[1]), max_high=self.max_high, fuse_1q=fuse)
         else:
-            prog, fmap, stats = compile_gpu(qc, k=kk, max_high=self.max_high, fuse_1q=fuse)
+            prog, fmap, stats = compile_gpu(qc, k=kk, max_high=self.max_high, fuse_1q=fuse,
+                                            cluster=0 if form == "plain" else compiler.GPU_CLUSTER)
         return prog, fmap, stats, 1, n, n, min(kk, n)
 
     def execute(self, prog, n_run, P, stream=None, timing=None):
