@@ -126,9 +126,11 @@ def test_compile_gpu_streams_vs_dense(n):
     for qc in [zoo(n, 5), C.make("qft", n), C.make("rqc", n), C.u3_cx_layers(n, 6, 4), C.cp_ladder(n)]:
         ref = dense.simulate(qc.tuples(), n)
         for k in (4, 5, 6):
-            # window scheduling cluster by cluster (the default) and op by op, and in-order scheduling
+            # window scheduling cluster by cluster (forced: these circuits are below the size the
+            # default clusters from) and op by op, and in-order scheduling
             for sched, cluster in (("window", K.GPU_CLUSTER), ("window", 0), ("inorder", 0)):
-                prog, fmap, st = K.compile_gpu(qc, k=k, scheduler=sched, cluster=cluster)
+                prog, fmap, st = K.compile_gpu(qc, k=k, scheduler=sched, cluster=cluster, cluster_always=True)
+                assert st.clustered == (sched == "window" and cluster > 0)
                 out = model.run_program(prog, n, 1, fmap, ctype=np.complex128)
                 assert 1 - judge.fidelity(ref, out) < 1e-10, (qc.name, k, sched, cluster)
 

@@ -147,3 +147,29 @@ def test_dense_records_op_by_op_mux2_kernel(gpu, n, k):
     with libsvx.options(fuse="off", diag_tables=False, mux_engine="valu"):
         got = run_hip(s0.astype(np.complex64), seg)
     _check(ref, got, (n, k), seg_nops(seg))
+
+
+@pytest.mark.parametrize("n,recipe", [(16, "u3cx"), (18, "u3cx"), (16, "rqc"), (17, "qft")])
+def test_cluster_scheduled_mux2_stream_vs_golden_model(gpu, n, recipe):
+    """MUX2 streams whose passes were drawn cluster by cluster (compile_gpu's clustered schedule),
+    run with the library's default pass fusion (whole clusters become FP32 dense units): GPU c64
+    against the c128 golden model of the same records."""
+    import torch
+    from u55c_statevector_30q_amd import libsvx
+    qc = C.u3_cx_layers(n, 6, 7) if recipe == "u3cx" else C.make(recipe, n)
+    prog, fmap, st = K.compile_gpu(qc, cluster_always=True)
+    assert st.clustered
+    gold = model.run_program(prog, n, 1, fmap, ctype=np.complex128)
+    s = torch.empty(1 << n, dtype=torch.complex64, device="cuda")
+    libsvx.zero_state(s)
+    hs = [libsvx.prepare(w, libsvx.C64, n) for kind, w in prog]
+    units = sum(libsvx.prepared_pass_stats(h, p)["dense4"] for h in hs for p in range(libsvx.prepared_info(h)[0]))
+    for h in hs:
+        libsvx.run_prepared(s, h)
+    torch.cuda.synchronize()
+    for h in hs:
+        libsvx.release_prepared(h)
+    if recipe == "u3cx":
+        assert units > 0
+    got = model.unpermute(s.cpu().numpy(), n, fmap)
+    _check(gold, got, (n, recipe), prog_nops(prog))

@@ -519,7 +519,7 @@ def schedule_clusters(ops, n, k, stats, M, width, max_ops=GPU_MAX_OPS, max_cost=
 
 
 def compile_gpu(qc, k=GPU_K, max_high=None, fuse_1q=True, scheduler="window", max_ops=GPU_MAX_OPS,
-                max_cost=None, cluster=GPU_CLUSTER):
+                max_cost=None, cluster=GPU_CLUSTER, cluster_always=False):
     """MI355X retarget of compile_circuit: P = 1, L = n, LDS-width tile k, up to max_high gathered
     high qubits.  Returns (prog, final_map, stats) like compile_circuit.
 
@@ -530,7 +530,8 @@ def compile_gpu(qc, k=GPU_K, max_high=None, fuse_1q=True, scheduler="window", ma
     (stats.clustered says which).  The stream stays
     plain MUX2; with the clustered schedule the library's pass fusion finds whole clusters inside
     single passes (30-qubit U3+CX depth 20: 178 ms per step against 192 ms op by op).  cluster = 0
-    schedules op by op, which suits a stream run with library fusion off."""
+    schedules op by op, which suits a stream run with library fusion off; cluster_always takes the
+    clustered schedule at any size and pass count."""
     n = qc.num_qubits
     k = min(k, n)
     if max_high is None:
@@ -542,9 +543,9 @@ def compile_gpu(qc, k=GPU_K, max_high=None, fuse_1q=True, scheduler="window", ma
     prog = []
     if scheduler == "window" and not wide:
         passes = schedule_window(ops, n, k, stats, max_high, max_ops, max_cost)
-        if cluster and passes and n >= CLUSTER_MIN_QUBITS:
+        if cluster and passes and (n >= CLUSTER_MIN_QUBITS or cluster_always):
             by_cluster = schedule_clusters(ops, n, k, stats, max_high, cluster, max_ops, max_cost)
-            if len(by_cluster) <= len(passes) * (1 + CLUSTER_MAX_EXTRA_PASSES):
+            if cluster_always or len(by_cluster) <= len(passes) * (1 + CLUSTER_MAX_EXTRA_PASSES):
                 passes = by_cluster
                 stats.clustered = True
         if passes:
