@@ -633,6 +633,7 @@ def main():
                                "(26..29 paired with 0..25)",
                    "n_qubits": n, "ops": st.n_ops, "passes": st.n_passes, "launches": n_launch,
                    "tile_k": compiler.GPU_K, "compile_s": round(t_compile, 3),
+                   "schedule": "clusters" if st.clustered else "ops",
                    "tile_order_tuned": {"xcd_contiguous": orders.count(-2), "identity": orders.count(0)}},
         "frac_of_peak": round(value / PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_GBS, "unit": "GB/s",
